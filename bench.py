@@ -1,0 +1,259 @@
+#!/usr/bin/env python3
+"""HipFull headline benchmark (one MI355X).
+
+Metric: field evaluations per second at 2^20 dipoles, p = 8, K = 32 moment
+states -- the MFMA-batched evaluate, reported with the plain-HIP FP32 rate
+beside it and the accuracy of the MFMA result.
+
+A step is one batched evaluate of K = 32 moment states on the fixed plan
+(2^20 dipoles uniform in [0,1)^3, targets = sources with the identity map,
+expansion order 8, uniform depth 5, FP32 plan, M2L on the matrix cores with
+bf16 hi/lo split operands).  Moments and fields stay in HBM (torch tensors);
+value = N * K * steps / elapsed seconds.  The plain path (M2L on the vector
+ALUs in FP32, one moment state per evaluate) is timed on the same plan.
+
+Accuracy: rel. L2 of the MFMA result against the FP64 FMM at the same (p,
+depth) (the HipFull FP64 plan, itself checked against CpuStatic to 1e-12 by
+the tests) and against the FP64 direct sum on 4096 seeded sample targets.
+
+Roofline: the dominant kernel is the leaf-level M2L product; its algorithmic
+work is 2 n^2 flops per list-2 interaction and moment state (n = 165 at
+p = 8), timed with HIP events on the plan's stream around that launch.
+
+CPU baseline: the repository's CpuStatic restatement (oracle/, "port") on a
+bounded sample of the same workload -- 2^17 dipoles at the same density
+(32 per leaf, depth 4), p = 8, one FP32 moment state -- timed on the host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "dip-fmm_amd"))
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed)
+SEED = 314160                   # the reference benchmark seed 314159 + config index 1
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def uniform_geometry(n: int, seed: int) -> np.ndarray:
+    return np.random.default_rng(seed).uniform(0.0, 1.0, size=(n, 3))
+
+
+def moment_states(n: int, k: int, seed: int) -> np.ndarray:
+    out = np.empty((k, n, 3), dtype=np.float32)
+    for i in range(k):
+        out[i] = np.random.default_rng(seed + 1000 + i).standard_normal((n, 3), dtype=np.float32)
+    return out
+
+
+def make_plan(hip, x, order, depth, precision, mode):
+    o = hip.UniformFmmOptions()
+    o.expansion_order = order
+    o.tree.max_level = depth
+    o.backend = hip.ExecutionBackend.HIP_FULL
+    o.precision = precision
+    o.fixed_target_source_indices = list(range(len(x)))
+    o.hip_m2l_mode = mode
+    return hip.UniformFmm(x, x, o)
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_baseline(order: int) -> dict:
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle
+
+    n, depth = 1 << 17, 4
+    x = uniform_geometry(n, SEED)
+    m = np.random.default_rng(SEED + 1000).standard_normal((n, 3))
+    ids = np.arange(n)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    f = oracle.CpuFmm(x, x, order, depth, 32, fixed_ids=ids)
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    f.evaluate(m)
+    dt = time.perf_counter() - t0
+    log(f"cpu baseline: setup {setup:.1f}s, evaluate {dt:.2f}s on {threads} threads")
+    return {
+        "value": n / dt,
+        "unit": "field evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"CpuStatic restatement, FP32, 2^17 uniform dipoles (32 per leaf, depth {depth}), p={order}, "
+                  f"1 moment state, one evaluate after plan setup ({setup:.0f}s setup, {dt:.2f}s evaluate)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--order", type=int, default=8)
+    ap.add_argument("--depth", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the HipFull evaluate runs on a single MI355X and this build has no "
+              "multi-GPU path; run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+    import cdfmm_hip as hip
+
+    if not hip.hip_full_available():
+        raise RuntimeError("HipFull needs a gfx950 device")
+    dev = torch.device("cuda:0")
+    N, K, p, depth = args.n, args.batch, args.order, args.depth
+    log(f"device: {hip.hip_device_description()}")
+    t0 = time.perf_counter()
+    x = uniform_geometry(N, SEED)
+    fmm = make_plan(hip, x, p, depth, hip.StaticPrecision.FLOAT32, hip.HipM2LMode.MFMA_BF16X3)
+    log(f"plan setup: {time.perf_counter() - t0:.1f}s")
+    stats = fmm.hip_plan_statistics
+    moments = torch.from_numpy(moment_states(N, K, SEED)).to(dev)
+    fields = torch.empty((K, N, 3), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        fmm.evaluate_device(moments.data_ptr(), fields.data_ptr(), K, stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    leaf = []
+    m2l = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        t = fmm.device_timings()  # waits for this evaluate; records its stage events
+        leaf.append(t["m2l_leaf"])
+        m2l.append(t["m2l"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = N * K * args.steps / elapsed
+    stage = fmm.device_timings()
+    log(f"MFMA K={K}: {ms_per_step:.2f} ms/step, {value:.3e} evals/s; stages {stage}")
+
+    leaf_pairs = stats["m2l_leaf_interaction_count"]
+    n = fmm.coefficient_count
+    leaf_flops = 2.0 * n * n * leaf_pairs * K
+    leaf_s = float(np.mean(leaf))
+    achieved = leaf_flops / leaf_s / 1e12
+    roofline = {
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": MFMA_BF16_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / MFMA_BF16_PEAK_TFLOPS,
+        "traffic": None,
+        "kernel": "m2l_mfma_kernel (leaf level)",
+        "kernel_ms": 1e3 * leaf_s,
+        "algorithmic_flops_per_launch": leaf_flops,
+        "split_products_per_flop": 3,
+    }
+
+    result_fields = fields[0].float().cpu().numpy()
+    plain = {}
+    if not args.no_plain:
+        fmm.set_m2l_mode(hip.HipM2LMode.VALU)
+        one_m = moments[0:1].contiguous()
+        one_f = torch.empty((1, N, 3), dtype=torch.float32, device=dev)
+        fmm.evaluate_device(one_m.data_ptr(), one_f.data_ptr(), 1, stream)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for _ in range(args.plain_steps):
+            fmm.evaluate_device(one_m.data_ptr(), one_f.data_ptr(), 1, stream)
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - tp) / args.plain_steps
+        pt = fmm.device_timings()
+        plain = {"value": N / plain_s, "ms_per_evaluate": 1e3 * plain_s, "m2l_leaf_ms": 1e3 * pt["m2l_leaf"]}
+        plain_fields = one_f[0].cpu().numpy()
+        fmm.set_m2l_mode(hip.HipM2LMode.MFMA_BF16X3)
+        log(f"plain VALU K=1: {1e3 * plain_s:.2f} ms/evaluate, {N / plain_s:.3e} evals/s")
+
+    accuracy = {}
+    if not args.no_accuracy:
+        del moments, fields
+        torch.cuda.empty_cache()
+        f64 = make_plan(hip, x, p, depth, hip.StaticPrecision.FLOAT64, hip.HipM2LMode.VALU)
+        m0 = moment_states(N, 1, SEED)[0].astype(np.float64)
+        ref64 = f64.evaluate(m0)["H"]
+        accuracy["mfma_vs_fp64_fmm"] = rel_l2(result_fields, ref64)
+        if plain:
+            accuracy["plain_fp32_vs_fp64_fmm"] = rel_l2(plain_fields, ref64)
+        del f64
+        sample = np.random.default_rng(SEED + 7).choice(N, size=4096, replace=False)
+        import ctypes
+
+        lib = ctypes.CDLL(os.path.join(HERE, "dip-fmm_amd", "libhipfmm.so"))
+        dp = ctypes.POINTER(ctypes.c_double)
+        tgt = np.ascontiguousarray(x[sample])
+        ids = np.ascontiguousarray(sample.astype(np.int32))
+        direct = np.zeros((4096, 3))
+        rc = lib.hipfmm_direct_field(tgt.ctypes.data_as(dp), 4096, x.ctypes.data_as(dp), N, m0.ctypes.data_as(dp),
+                                     ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), direct.ctypes.data_as(dp))
+        if rc != 0:
+            raise RuntimeError("GPU direct sum failed")
+        accuracy["mfma_vs_fp64_direct_4096"] = rel_l2(result_fields[sample], direct)
+        accuracy["fp64_fmm_vs_fp64_direct_4096"] = rel_l2(ref64[sample], direct)
+        log(f"accuracy: {accuracy}")
+
+    baseline = None if args.no_cpu_baseline else cpu_baseline(p)
+
+    line = {
+        "metric": "field evals/s, 1M dipoles p=8 K=32: MFMA-batched vs plain HIP @ rel err≤1e-5",
+        "value": value,
+        "unit": "field evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: uniform [0,1)^3 positions, N(0,1) moment components, seeded",
+        "config": {
+            "workload": f"{N} dipoles, targets = sources (identity map), p={p}, uniform depth {depth}, "
+                        f"K={K} moment states per evaluate, FP32 plan, M2L on MFMA (bf16 hi/lo x3, fp32 accumulate)",
+            "n": N, "order": p, "depth": depth, "batch": K, "m2l": "mfma_bf16x3",
+        },
+        "plain_hip_fp32": plain or None,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "accuracy": accuracy or None,
+        "stages_ms": {k: 1e3 * v for k, v in stage.items() if k != "evaluations"},
+        "roofline": roofline,
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,89 @@
+"""Build the HipFull libraries in-tree.
+
+Produces, next to this file:
+  libhipfmm.so                      HIP kernels + C-ABI (include/hipfmm.h), gfx950
+  _cdfmm_hip<EXT_SUFFIX>            pybind11 extension: host front-end (tree, plan,
+                                    UniformFmm) linked to libhipfmm.so, imported
+                                    through cdfmm_hip.py
+
+Run ``python dip-fmm_amd/build.py`` or call ``build()``.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+BUILD_DIR = os.path.join(HERE, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX_HOST", "g++")
+
+# Packed FP32 instructions are disabled for all device code: the compiler
+# still emits plain FP32 VALU ops.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-result",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+HIP_SOURCES = ["plan.hip", "far.hip", "near.hip", "m2l_valu.hip", "m2l_mfma.hip"]
+HOST_SOURCES = ["host/operators.cpp", "host/uniform_fmm.cpp", "python/bindings.cpp"]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
+
+
+def hip_library() -> str:
+    return os.path.join(HERE, "libhipfmm.so")
+
+
+def module_path() -> str:
+    return os.path.join(HERE, "_cdfmm_hip" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build(jobs: int = 8) -> None:
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    csrc = os.path.join(HERE, "csrc")
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(csrc, "hip", src), "-o", obj])
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+        for fut in [pool.submit(_run, c) for c in cmds]:
+            fut.result()
+    lib_tmp = hip_library() + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", lib_tmp])
+    os.replace(lib_tmp, hip_library())
+
+    import pybind11
+
+    mod_tmp = module_path() + ".tmp"
+    _run([
+        CXX, "-std=c++20", "-O3", "-fPIC", "-fopenmp", "-shared", "-Wall", "-Wno-unknown-pragmas",
+        *[os.path.join(csrc, s) for s in HOST_SOURCES],
+        "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+        "-L", HERE, "-lhipfmm", "-Wl,-rpath,$ORIGIN", "-o", mod_tmp,
+    ])
+    os.replace(mod_tmp, module_path())
+
+
+def import_module():
+    """Import the built ``cdfmm_hip`` module from this directory."""
+    if HERE not in sys.path:
+        sys.path.insert(0, HERE)
+    import cdfmm_hip  # noqa: E402
+
+    return cdfmm_hip
+
+
+if __name__ == "__main__":
+    build()
+    print(module_path())

@@ -1,0 +1,157 @@
+// Internal declarations shared by the HipFull device translation units.
+//
+// Device data layout (HBM), for a plan of order p (n coefficients), leaf
+// level L, N_s sources, N_t targets, batch K moment states:
+//   positions     sorted, split as float4 hi + float4 lo (FP32 plans) or
+//                 double4 (FP64 plans); plus leaf-relative offsets (float4 /
+//                 double4) rounded once from FP64 on the host
+//   moments       sm[source][3][K]        (sorted, k fastest)
+//   multipoles    M[node][n][K]           (node-major, level by level)
+//   locals        Lc[node][n][K]
+//   near field    Hn[target][3][K]        (sorted)
+//   fields out    out[k][user target][3]
+// Boxes of one level are addressed by their Morton index; a child's node is
+// level_offset(l+1) + 8*parent_morton + class, so the 8 children of a parent
+// are contiguous and the list-1/list-2 neighbourhoods are enumerated from
+// integer box offsets instead of stored lists.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hipfmm {
+
+constexpr int kMaxBatch = 64;
+constexpr int kMaxOrder = 10;
+constexpr int kOffsetSlots = 343;  // (dx,dy,dz) in [-3,3]^3
+
+struct hip_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct invalid_argument : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+
+#define HIPFMM_CHECK(expr)                                                                   \
+  do {                                                                                       \
+    const hipError_t hipfmm_err_ = (expr);                                                   \
+    if (hipfmm_err_ != hipSuccess) {                                                         \
+      (void)hipGetLastError(); /* clear the sticky copy so later checks see fresh state */   \
+      throw ::hipfmm::hip_error(std::string(#expr) + ": " + hipGetErrorString(hipfmm_err_)); \
+    }                                                                                        \
+  } while (0)
+
+__host__ __device__ inline int coefficient_count(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+__host__ __device__ inline int64_t level_offset(int level) { return ((int64_t(1) << (3 * level)) - 1) / 7; }
+__host__ __device__ inline int offset_slot(int dx, int dy, int dz) { return (dx + 3) * 49 + (dy + 3) * 7 + (dz + 3); }
+
+__host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+  uint32_t m = 0;
+  for (int b = 0; b < bits; ++b)
+    m |= (((x >> b) & 1u) << (3 * b)) | (((y >> b) & 1u) << (3 * b + 1)) | (((z >> b) & 1u) << (3 * b + 2));
+  return m;
+}
+__host__ __device__ inline void demorton3(uint32_t m, int bits, int& x, int& y, int& z) {
+  x = y = z = 0;
+  for (int b = 0; b < bits; ++b) {
+    x |= int((m >> (3 * b)) & 1u) << b;
+    y |= int((m >> (3 * b + 1)) & 1u) << b;
+    z |= int((m >> (3 * b + 2)) & 1u) << b;
+  }
+}
+
+template <class S>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
+// Sparse shift operator (M2M or L2L) shared by all levels and child classes:
+// CSR over output coefficients, values per (level, class).
+struct ShiftCsr {
+  int nnz = 0;
+  int* row_offsets = nullptr;  // n+1
+  int* columns = nullptr;      // nnz
+  void* values = nullptr;      // (L+1) x 8 x nnz scalars (levels 1..L used)
+};
+
+// Everything the kernels read, with device pointers.
+struct DevicePlan {
+  int precision = 32;
+  int p = 0, n = 0, nm = 0;  // nm: coefficients of order p-1 (monomials used by P2M/L2P)
+  int L = 0;
+  int ns = 0, nt = 0;
+  int64_t nodes = 0, leaves = 0;
+  // geometry
+  void* src_pos = nullptr;  // FP32: 2*ns float4 (hi block then lo block); FP64: ns double4
+  void* tgt_pos = nullptr;
+  void* src_rel = nullptr;  // ns float4/double4 leaf-relative offsets
+  void* tgt_rel = nullptr;
+  int* src_perm = nullptr;
+  int* tgt_perm = nullptr;
+  int* self = nullptr;      // nt sorted self indices (-1 none)
+  int2* leaf_src = nullptr; // per leaf [begin, end)
+  int2* leaf_tgt = nullptr;
+  // basis tables
+  int4* p2m_tab = nullptr;  // per alpha: index(alpha-e_c) or -1 for c=0..2, w = sign (+1/-1)
+  // operators
+  void* m2l_T = nullptr;     // FP32/FP64 class matrices, column-major [class][alpha][beta]
+  int* m2l_slot = nullptr;   // 343 slot -> class index or -1
+  void* ms = nullptr;        // (L+1) x n
+  void* ls = nullptr;
+  ShiftCsr m2m, l2l;
+  // state (capacity K)
+  int kcap = 0;
+  void* moments_in = nullptr;  // host-path staging: K x ns x 3
+  void* fields_out = nullptr;  // host-path staging: K x nt x 3
+  void* sm = nullptr;
+  void* M = nullptr;
+  void* Lc = nullptr;
+  void* Hn = nullptr;
+  // MFMA operands
+  void* mt_hi = nullptr;  // bf16/fp16 [node][K][apad]
+  void* mt_lo = nullptr;
+  void* T_hi = nullptr;   // [class][bpad][apad] 16-bit, MFMA fragment order
+  void* T_lo = nullptr;
+  void* ls_pad = nullptr; // (L+1) x bpad float local scaling (0 beyond n)
+  float* lvl_scale = nullptr; // per level operand scale for FP16 mode
+  int apad = 0, bpad = 0;
+};
+
+// Kernel launchers (one translation unit each).
+template <class S>
+void launch_gather(const DevicePlan& d, const S* user_moments, S scale, int K, hipStream_t s);
+template <class S>
+void launch_p2m(const DevicePlan& d, int K, hipStream_t s);
+template <class S>
+void launch_m2m(const DevicePlan& d, int parent_level, int K, hipStream_t s);
+template <class S>
+void launch_l2l(const DevicePlan& d, int child_level, int K, hipStream_t s);
+template <class S>
+void launch_m2l_valu(const DevicePlan& d, int level, int K, hipStream_t s);
+template <class S>
+void launch_p2p(const DevicePlan& d, int K, hipStream_t s);
+template <class S>
+void launch_l2p_unsort(const DevicePlan& d, S* fields, int K, hipStream_t s);
+template <class S>
+void launch_zero(S* ptr, size_t count, hipStream_t s);
+
+void launch_m2l_split(const DevicePlan& d, int level, int K, int mode, hipStream_t s);
+void launch_m2l_mfma(const DevicePlan& d, int level, int K, int mode, hipStream_t s);
+void prepare_mfma_operators(DevicePlan& d, const std::vector<double>& matrices, int classes,
+                            const std::vector<int>& slot_of_class, const std::vector<double>& local_scaling,
+                            std::vector<void*>& allocations, size_t& bytes);
+
+void direct_field_fp64(const double* targets, int nt, const double* sources, int ns, const double* moments,
+                       const int* self, double* fields);
+
+}  // namespace hipfmm

@@ -1,0 +1,227 @@
+// List-1 P2P (near field) and the FP64 direct sum.
+//
+// Reference: the list-1 pair set of src/uniform_fmm.cpp:678-710 (every
+// source of the clipped 3x3x3 leaf neighbourhood of the target's leaf, the
+// identity pair skipped, coincident non-identity pairs giving NaN:
+// src/static_operators.cpp:435-446) with the point-dipole tensor
+// D = (3 r r^T/|r|^5 - I/|r|^3)/(4 pi) of src/cuboid.cpp:293-307.  The
+// reference stores one tensor per pair (37 GB at 2^20 points); here each
+// workgroup owns one target leaf, streams the neighbour leaves' sources
+// through LDS and evaluates the tensor on the fly, reusing every staged
+// source for all K moment states.
+//
+// FP32 plans carry positions as hi+lo float pairs of the FP64 operator-frame
+// coordinates, so r = (t_hi - s_hi) + (t_lo - s_lo) keeps FP32 relative
+// accuracy even for pairs far closer than a leaf width.  The field is formed
+// as f*(3(u.m)u - m) with u = r/|r| and f = |r|^-3/(4 pi), which avoids the
+// |r|^-5 overflow of close pairs in FP32.
+#include "common.hpp"
+
+namespace hipfmm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kSlots = 32;  // target slots per pass; 8 thread groups per slot
+
+template <class S>
+struct PairPos;
+
+template <>
+struct PairPos<float> {
+  float4 hi, lo;
+  __device__ static PairPos load(const void* pos, int count, int i) {
+    const float4* p = static_cast<const float4*>(pos);
+    return {p[i], p[count + i]};
+  }
+  __device__ inline void diff(const PairPos& s, float& x, float& y, float& z) const {
+    x = (hi.x - s.hi.x) + (lo.x - s.lo.x);
+    y = (hi.y - s.hi.y) + (lo.y - s.lo.y);
+    z = (hi.z - s.hi.z) + (lo.z - s.lo.z);
+  }
+};
+
+template <>
+struct PairPos<double> {
+  double4 hi;
+  __device__ static PairPos load(const void* pos, int, int i) { return {static_cast<const double4*>(pos)[i]}; }
+  __device__ inline void diff(const PairPos& s, double& x, double& y, double& z) const {
+    x = hi.x - s.hi.x;
+    y = hi.y - s.hi.y;
+    z = hi.z - s.hi.z;
+  }
+};
+
+__device__ inline float inv_sqrt(float x) { return __frsqrt_rn(x); }
+__device__ inline double inv_sqrt(double x) { return 1.0 / sqrt(x); }
+
+// Workgroup per target leaf.  Thread = (slot, group); group g handles the k
+// range [kg*KPT, kg*KPT+KPT) and the sources j with j % SP == sp, where
+// KG = number of k groups and SP = 8 / KG source splits.
+template <class S, int KPT>
+__global__ void __launch_bounds__(kThreads) p2p_kernel(const DevicePlan d, int K, int KG, int chunk) {
+  extern __shared__ unsigned char smem_raw[];
+  auto* spos = reinterpret_cast<PairPos<S>*>(smem_raw);       // chunk
+  S* smom = reinterpret_cast<S*>(spos + chunk);                // chunk x 3 x K
+  S* red = smom;                                               // reused for the split reduction
+  const int leaf = blockIdx.x;
+  const int2 trange = d.leaf_tgt[leaf];
+  if (trange.x == trange.y) return;
+  const int slot = threadIdx.x % kSlots, g = threadIdx.x / kSlots;
+  const int SP = 8 / KG;
+  const int kg = g % KG, sp = g / KG;
+  const int nb = 1 << d.L;
+  int lx, ly, lz;
+  demorton3(uint32_t(leaf), d.L, lx, ly, lz);
+  const S c4pi = S(0.079577471545947667884441881686257181);  // 1/(4 pi)
+  const S* sm = static_cast<const S*>(d.sm);
+  S* Hn = static_cast<S*>(d.Hn);
+  for (int t0 = trange.x; t0 < trange.y; t0 += kSlots) {
+    const int t = t0 + slot;
+    const bool active = t < trange.y;
+    const PairPos<S> tp = PairPos<S>::load(d.tgt_pos, d.nt, active ? t : trange.x);
+    const int self = active ? d.self[t] : -2;
+    S acc[KPT][3];
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) acc[q][0] = acc[q][1] = acc[q][2] = S(0);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int x = lx + dx, y = ly + dy, z = lz + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+          const int2 sr = d.leaf_src[morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L)];
+          for (int j0 = sr.x; j0 < sr.y; j0 += chunk) {
+            const int cnt = min(chunk, sr.y - j0);
+            __syncthreads();
+            for (int w = threadIdx.x; w < cnt; w += kThreads) spos[w] = PairPos<S>::load(d.src_pos, d.ns, j0 + w);
+            for (int w = threadIdx.x; w < cnt * 3 * K; w += kThreads) smom[w] = sm[int64_t(j0) * 3 * K + w];
+            __syncthreads();
+            if (active) {
+              for (int j = sp; j < cnt; j += SP) {
+                if (j0 + j == self) continue;
+                S rx, ry, rz;
+                tp.diff(spos[j], rx, ry, rz);
+                const S r2 = rx * rx + ry * ry + rz * rz;
+                const S rinv = inv_sqrt(r2);
+                const S f = rinv * rinv * rinv * c4pi;
+                const S ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
+                const S* mj = smom + j * 3 * K + kg * KPT;
+#pragma unroll
+                for (int q = 0; q < KPT; ++q) {
+                  if (kg * KPT + q < K) {
+                    const S mx = mj[q], my = mj[K + q], mz = mj[2 * K + q];
+                    const S mu3 = S(3) * (ux * mx + uy * my + uz * mz);
+                    acc[q][0] += f * (mu3 * ux - mx);
+                    acc[q][1] += f * (mu3 * uy - my);
+                    acc[q][2] += f * (mu3 * uz - mz);
+                  }
+                }
+              }
+            }
+          }
+        }
+    // Reduce the SP source splits (fixed order) and store Hn[t][c][k].
+    __syncthreads();
+    if (SP > 1) {
+      for (int q = 0; q < KPT; ++q)
+        for (int c = 0; c < 3; ++c) red[((g * kSlots + slot) * KPT + q) * 3 + c] = acc[q][c];
+      __syncthreads();
+      if (sp == 0 && active) {
+        for (int s2 = 1; s2 < SP; ++s2) {
+          const int g2 = s2 * KG + kg;
+          for (int q = 0; q < KPT; ++q)
+            for (int c = 0; c < 3; ++c) acc[q][c] += red[((g2 * kSlots + slot) * KPT + q) * 3 + c];
+        }
+      }
+    }
+    if (sp == 0 && active) {
+      for (int q = 0; q < KPT; ++q) {
+        const int k = kg * KPT + q;
+        if (k < K)
+          for (int c = 0; c < 3; ++c) Hn[(int64_t(t) * 3 + c) * K + k] = acc[q][c];
+      }
+    }
+  }
+}
+
+template <class S, int KPT>
+void launch_p2p_t(const DevicePlan& d, int K, hipStream_t s) {
+  int KG = (K + KPT - 1) / KPT;
+  int kg = 1;
+  while (kg < KG) kg *= 2;  // 1, 2, 4 or 8 k groups
+  const int chunk = sizeof(S) == 4 ? 64 : 32;
+  const size_t stage = size_t(chunk) * sizeof(PairPos<S>) + size_t(chunk) * 3 * K * sizeof(S);
+  const size_t reduce = size_t(8) * kSlots * KPT * 3 * sizeof(S) + size_t(chunk) * sizeof(PairPos<S>);
+  p2p_kernel<S, KPT><<<unsigned(d.leaves), kThreads, std::max(stage, reduce), s>>>(d, K, kg, chunk);
+  HIPFMM_CHECK(hipGetLastError());
+}
+
+// FP64 direct field, one thread per target, sources staged through LDS.
+// validation.cpp:71-111 / operators.cpp:178-220; model kernel
+// src/cuda_fmm.cu:518-569.
+__global__ void __launch_bounds__(kThreads) direct_kernel(const double* __restrict__ tgt, int nt,
+                                                          const double* __restrict__ src, int ns,
+                                                          const double* __restrict__ mom, const int* __restrict__ self,
+                                                          double* __restrict__ out) {
+  __shared__ double s_pos[kThreads * 3];
+  __shared__ double s_mom[kThreads * 3];
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  const bool active = t < nt;
+  const double tx = active ? tgt[3 * t] : 0.0, ty = active ? tgt[3 * t + 1] : 0.0, tz = active ? tgt[3 * t + 2] : 0.0;
+  const int me = active && self ? self[t] : -1;
+  const double c = 0.079577471545947667884441881686257181;
+  double hx = 0.0, hy = 0.0, hz = 0.0;
+  for (int j0 = 0; j0 < ns; j0 += kThreads) {
+    const int cnt = min(kThreads, ns - j0);
+    __syncthreads();
+    for (int w = threadIdx.x; w < 3 * cnt; w += kThreads) {
+      s_pos[w] = src[3 * j0 + w];
+      s_mom[w] = mom[3 * j0 + w];
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int j = 0; j < cnt; ++j) {
+      if (j0 + j == me) continue;
+      const double rx = tx - s_pos[3 * j], ry = ty - s_pos[3 * j + 1], rz = tz - s_pos[3 * j + 2];
+      const double r2 = rx * rx + ry * ry + rz * rz;
+      const double rinv = 1.0 / sqrt(r2);
+      const double rinv3 = rinv * rinv * rinv;
+      const double rinv5 = rinv3 * rinv * rinv;
+      const double mx = s_mom[3 * j], my = s_mom[3 * j + 1], mz = s_mom[3 * j + 2];
+      const double mdr = mx * rx + my * ry + mz * rz;
+      hx += (rx * (3.0 * mdr * rinv5) - mx * rinv3) * c;
+      hy += (ry * (3.0 * mdr * rinv5) - my * rinv3) * c;
+      hz += (rz * (3.0 * mdr * rinv5) - mz * rinv3) * c;
+    }
+  }
+  if (active) {
+    out[3 * t] = hx;
+    out[3 * t + 1] = hy;
+    out[3 * t + 2] = hz;
+  }
+}
+
+}  // namespace
+
+template <class S>
+void launch_p2p(const DevicePlan& d, int K, hipStream_t s) {
+  if (d.leaves == 0 || d.nt == 0) return;
+  if (K <= 8)
+    launch_p2p_t<S, 1>(d, K, s);
+  else if (K <= 32)
+    launch_p2p_t<S, 4>(d, K, s);
+  else
+    launch_p2p_t<S, 8>(d, K, s);
+}
+template void launch_p2p<float>(const DevicePlan&, int, hipStream_t);
+template void launch_p2p<double>(const DevicePlan&, int, hipStream_t);
+
+void direct_field_fp64(const double* targets, int nt, const double* sources, int ns, const double* moments,
+                       const int* self, double* fields) {
+  if (nt == 0) return;
+  direct_kernel<<<unsigned((nt + kThreads - 1) / kThreads), kThreads, 0, nullptr>>>(targets, nt, sources, ns,
+                                                                                      moments, self, fields);
+  HIPFMM_CHECK(hipGetLastError());
+}
+
+}  // namespace hipfmm

@@ -1,0 +1,748 @@
+// HipFull device plan: C-ABI implementation (include/hipfmm.h).
+//
+// Replaces the reference's CudaFullPlan (src/cuda_fmm_plan.hpp:95-114,
+// src/cuda_fmm.cu:2636-3665).  Static data is uploaded once at create; each
+// evaluate enqueues
+//   far stream : gather -> P2M -> M2M (levels L-1..2) -> M2L (levels 2..L)
+//                -> L2L (levels 3..L) -> [wait near] -> L2P + combine + unsort
+//   near stream: [wait gather] -> list-1 P2P
+// and, for host operands, exactly one H2D (moments) and one D2H (fields),
+// matching the reference's transfer contract (tests/test_cuda_backend.cpp:
+// 102-105 there).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/hipfmm.h"
+#include "common.hpp"
+
+using hipfmm::DevicePlan;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+enum Ev {
+  kEvH2D0,
+  kEvH2D1,
+  kEvGather,
+  kEvP2M,
+  kEvM2M,
+  kEvM2L,
+  kEvL2L,
+  kEvL2P,
+  kEvNear0,
+  kEvNear1,
+  kEvD2H0,
+  kEvD2H1,
+  kEvUser,
+  kEvLeafM2L0,
+  kEvLeafM2L1,
+  kEvCount
+};
+
+}  // namespace
+
+struct hipfmm_plan {
+  DevicePlan d;
+  int device = 0;
+  std::vector<void*> persistent;
+  std::vector<void*> state;
+  hipStream_t s_far = nullptr, s_near = nullptr;
+  hipEvent_t ev[kEvCount] = {};
+  void* pinned_in = nullptr;
+  void* pinned_out = nullptr;
+  size_t pinned_in_bytes = 0, pinned_out_bytes = 0;
+  std::vector<int> fixed_self;
+  bool identity_initialised = false;
+  double device_moment_scale = 1.0;
+  int m2l_mode = HIPFMM_M2L_VALU;
+  bool mfma_ready = false;
+  hipfmm_stats stats{};
+  hipfmm_timings timings{};
+  bool timings_pending = false;
+  bool last_host = false;
+  int last_K = 0;
+  size_t persistent_bytes = 0;
+  size_t state_bytes = 0;
+};
+
+namespace {
+
+struct DeviceGuard {
+  int previous = -1;
+  explicit DeviceGuard(int device) {
+    HIPFMM_CHECK(hipGetDevice(&previous));
+    if (previous != device) HIPFMM_CHECK(hipSetDevice(device));
+  }
+  ~DeviceGuard() {
+    int now = -1;
+    if (hipGetDevice(&now) == hipSuccess && now != previous && previous >= 0) (void)hipSetDevice(previous);
+  }
+};
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return HIPFMM_OK;
+  } catch (const hipfmm::invalid_argument& e) {
+    g_last_error = e.what();
+    return HIPFMM_ERROR_INVALID_ARGUMENT;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return HIPFMM_ERROR_INVALID_ARGUMENT;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return HIPFMM_ERROR_RUNTIME;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return HIPFMM_ERROR_RUNTIME;
+  }
+}
+
+void require(bool ok, const char* what) {
+  if (!ok) throw hipfmm::invalid_argument(what);
+}
+
+void* device_alloc(hipfmm_plan* P, size_t bytes, std::vector<void*>& list, size_t& counter) {
+  if (bytes == 0) bytes = 16;
+  void* ptr = nullptr;
+  HIPFMM_CHECK(hipMalloc(&ptr, bytes));
+  list.push_back(ptr);
+  counter += bytes;
+  return ptr;
+}
+
+template <class T>
+T* upload(hipfmm_plan* P, const T* host, size_t count) {
+  const size_t bytes = count * sizeof(T);
+  T* ptr = static_cast<T*>(device_alloc(P, bytes, P->persistent, P->persistent_bytes));
+  if (count && host) HIPFMM_CHECK(hipMemcpy(ptr, host, bytes, hipMemcpyHostToDevice));
+  if (count && !host) HIPFMM_CHECK(hipMemset(ptr, 0, bytes));  // empty population placeholder
+  P->stats.setup_h2d_bytes += bytes;
+  return ptr;
+}
+
+int gfx950_devices() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return count;
+}
+
+bool device_is_gfx950(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return false;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+// Basis tables: entry a -> (index(a-e_x), index(a-e_y), index(a-e_z), sign);
+// entry n+a -> (ax, ay, az, |a|).  include/cdfmm/multi_index.hpp:62-71.
+std::vector<int4> basis_tables(int p) {
+  std::vector<int4> exps;
+  for (int d = 0; d <= p; ++d)
+    for (int ax = 0; ax <= d; ++ax)
+      for (int ay = 0; ay <= d - ax; ++ay) exps.push_back(int4{ax, ay, d - ax - ay, d});
+  const int n = int(exps.size());
+  auto index_of = [&](int ax, int ay, int az) {
+    if (ax < 0 || ay < 0 || az < 0) return -1;
+    for (int i = 0; i < n; ++i)
+      if (exps[i].x == ax && exps[i].y == ay && exps[i].z == az) return i;
+    return -1;
+  };
+  std::vector<int4> tab(2 * size_t(n));
+  for (int a = 0; a < n; ++a) {
+    const int4 e = exps[a];
+    tab[a] = int4{index_of(e.x - 1, e.y, e.z), index_of(e.x, e.y - 1, e.z), index_of(e.x, e.y, e.z - 1),
+                  e.w % 2 == 0 ? 1 : -1};
+    tab[n + a] = e;
+  }
+  return tab;
+}
+
+template <class S>
+hipfmm::ShiftCsr shift_csr(hipfmm_plan* P, const double* dense, int L, int n) {
+  // Union sparsity pattern over levels 1..L and the 8 classes.
+  std::vector<char> nz(size_t(n) * n, 0);
+  for (int l = 1; l <= L; ++l)
+    for (int c = 0; c < 8; ++c) {
+      const double* m = dense + (size_t(l - 1) * 8 + c) * n * n;
+      for (size_t i = 0; i < size_t(n) * n; ++i)
+        if (m[i] != 0.0) nz[i] = 1;
+    }
+  std::vector<int> off(n + 1, 0), col;
+  for (int r = 0; r < n; ++r) {
+    for (int cidx = 0; cidx < n; ++cidx)
+      if (nz[size_t(r) * n + cidx]) col.push_back(cidx);
+    off[r + 1] = int(col.size());
+  }
+  const int nnz = int(col.size());
+  std::vector<S> vals(size_t(L + 1) * 8 * std::max(nnz, 1), S(0));
+  for (int l = 1; l <= L; ++l)
+    for (int c = 0; c < 8; ++c) {
+      const double* m = dense + (size_t(l - 1) * 8 + c) * n * n;
+      S* v = vals.data() + (size_t(l) * 8 + c) * nnz;
+      for (int r = 0; r < n; ++r)
+        for (int e = off[r]; e < off[r + 1]; ++e) v[e] = S(m[size_t(r) * n + col[e]]);
+    }
+  hipfmm::ShiftCsr out;
+  out.nnz = nnz;
+  out.row_offsets = upload(P, off.data(), off.size());
+  out.columns = upload(P, col.data(), std::max<size_t>(col.size(), 1));
+  out.values = upload(P, vals.data(), vals.size());
+  return out;
+}
+
+template <class S>
+void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
+  DevicePlan& d = P->d;
+  const int p = desc.order, L = desc.max_level, n = hipfmm::coefficient_count(p);
+  d.precision = desc.precision;
+  d.p = p;
+  d.n = n;
+  d.nm = p >= 1 ? hipfmm::coefficient_count(p - 1) : 0;
+  d.L = L;
+  d.ns = desc.source_count;
+  d.nt = desc.target_count;
+  d.nodes = hipfmm::level_offset(L + 1);
+  d.leaves = int64_t(1) << (3 * L);
+  const int64_t leaf0 = hipfmm::level_offset(L);
+  const int64_t* ranges = desc.node_ranges;
+  // Geometry.
+  using V4 = typename hipfmm::Vec4<S>::type;
+  auto pack = [&](const double* pos, int count, void*& abs_out, void*& rel_out, bool is_src) {
+    std::vector<V4> rel(std::max(count, 1));
+    std::vector<int> leaf_of(std::max(count, 1), -1);
+    for (int64_t leaf = 0; leaf < d.leaves; ++leaf) {
+      const int64_t* r = ranges + 4 * (leaf0 + leaf) + (is_src ? 0 : 2);
+      require(r[0] >= 0 && r[1] >= r[0] && r[1] <= count, "node ranges are inconsistent with the point count");
+      for (int64_t i = r[0]; i < r[1]; ++i) leaf_of[size_t(i)] = int(leaf);
+    }
+    for (int i = 0; i < count; ++i) {
+      require(leaf_of[i] >= 0, "leaf ranges do not cover every point");
+      const double* c = desc.node_centres + 3 * (leaf0 + leaf_of[i]);
+      const double* x = pos + 3 * size_t(i);
+      rel[i] = V4{S(x[0] - c[0]), S(x[1] - c[1]), S(x[2] - c[2]), S(0)};
+    }
+    if constexpr (sizeof(S) == 4) {
+      std::vector<float4> hl(2 * size_t(std::max(count, 1)));
+      for (int i = 0; i < count; ++i) {
+        const double* x = pos + 3 * size_t(i);
+        const float hx = float(x[0]), hy = float(x[1]), hz = float(x[2]);
+        hl[i] = float4{hx, hy, hz, 0.0f};
+        hl[size_t(count) + i] =
+            float4{float(x[0] - double(hx)), float(x[1] - double(hy)), float(x[2] - double(hz)), 0.0f};
+      }
+      abs_out = upload(P, hl.data(), hl.size());
+    } else {
+      std::vector<double4> a(std::max(count, 1));
+      for (int i = 0; i < count; ++i) {
+        const double* x = pos + 3 * size_t(i);
+        a[i] = double4{x[0], x[1], x[2], 0.0};
+      }
+      abs_out = upload(P, a.data(), a.size());
+    }
+    rel_out = upload(P, rel.data(), rel.size());
+  };
+  pack(desc.sorted_sources, d.ns, d.src_pos, d.src_rel, true);
+  pack(desc.sorted_targets, d.nt, d.tgt_pos, d.tgt_rel, false);
+  d.src_perm = upload(P, desc.source_permutation, std::max(d.ns, 1));
+  d.tgt_perm = upload(P, desc.target_permutation, std::max(d.nt, 1));
+  std::vector<int2> ls(d.leaves), lt(d.leaves);
+  for (int64_t leaf = 0; leaf < d.leaves; ++leaf) {
+    const int64_t* r = ranges + 4 * (leaf0 + leaf);
+    ls[leaf] = int2{int(r[0]), int(r[1])};
+    lt[leaf] = int2{int(r[2]), int(r[3])};
+  }
+  d.leaf_src = upload(P, ls.data(), ls.size());
+  d.leaf_tgt = upload(P, lt.data(), lt.size());
+  std::vector<int> self(std::max(d.nt, 1), -1);
+  if (desc.fixed_self_indices) {
+    for (int t = 0; t < d.nt; ++t) {
+      require(desc.fixed_self_indices[t] >= -1 && desc.fixed_self_indices[t] < d.ns,
+              "fixed identity map contains an invalid index");
+      self[t] = desc.fixed_self_indices[t];
+    }
+    P->fixed_self.assign(self.begin(), self.begin() + d.nt);
+    P->identity_initialised = true;
+  }
+  d.self = upload(P, self.data(), self.size());
+  const auto tab = basis_tables(p);
+  d.p2m_tab = upload(P, tab.data(), tab.size());
+  // M2L class matrices keyed by offset slot.
+  std::vector<int> slot(hipfmm::kOffsetSlots, -1), slot_of_class(std::max(desc.m2l_class_count, 1), -1);
+  for (int c = 0; c < desc.m2l_class_count; ++c) {
+    const int* o = desc.m2l_class_offsets + 3 * c;
+    require(std::abs(o[0]) <= 3 && std::abs(o[1]) <= 3 && std::abs(o[2]) <= 3, "M2L class offset out of range");
+    const int s = hipfmm::offset_slot(o[0], o[1], o[2]);
+    slot[s] = c;
+    slot_of_class[c] = s;
+  }
+  if (L >= 2) {
+    for (int x = -3; x <= 3; ++x)
+      for (int y = -3; y <= 3; ++y)
+        for (int z = -3; z <= 3; ++z)
+          if (std::max({std::abs(x), std::abs(y), std::abs(z)}) >= 2)
+            require(slot[hipfmm::offset_slot(x, y, z)] >= 0, "an M2L class matrix for a list-2 offset is missing");
+  }
+  const size_t nn = size_t(n) * n;
+  std::vector<S> T(std::max<size_t>(size_t(desc.m2l_class_count) * nn, 1));
+  for (size_t i = 0; i < size_t(desc.m2l_class_count) * nn; ++i) T[i] = S(desc.m2l_matrices[i]);
+  d.m2l_T = upload(P, T.data(), T.size());
+  d.m2l_slot = upload(P, slot.data(), slot.size());
+  std::vector<S> ms(size_t(L + 1) * n), lsc(size_t(L + 1) * n);
+  for (size_t i = 0; i < ms.size(); ++i) {
+    ms[i] = S(desc.m2l_multipole_scaling[i]);
+    lsc[i] = S(desc.m2l_local_scaling[i]);
+  }
+  d.ms = upload(P, ms.data(), ms.size());
+  d.ls = upload(P, lsc.data(), lsc.size());
+  if (L >= 1) {
+    d.m2m = shift_csr<S>(P, desc.m2m_operators, L, n);
+    d.l2l = shift_csr<S>(P, desc.l2l_operators, L, n);
+  }
+  P->stats.scalar_bytes = sizeof(S);
+  P->stats.m2m_unique_matrix_count = uint64_t(L) * 8;
+  P->stats.m2m_matrix_bytes = uint64_t(L) * 8 * d.m2m.nnz * sizeof(S);
+  P->stats.l2l_unique_matrix_count = uint64_t(L) * 8;
+  P->stats.l2l_matrix_bytes = uint64_t(L) * 8 * d.l2l.nnz * sizeof(S);
+  P->stats.m2l_unique_matrix_count = uint64_t(desc.m2l_class_count);
+  P->stats.m2l_matrix_bytes = uint64_t(desc.m2l_class_count) * nn * sizeof(S);
+  // Interaction counts of the canonical plan (non-empty pairs only).
+  uint64_t m2l_pairs = 0, p2p_pairs = 0, leaf_pairs = 0;
+  for (int level = 2; level <= L; ++level) {
+    const int nb = 1 << level;
+    const int64_t base = hipfmm::level_offset(level);
+    for (int64_t m = 0; m < (int64_t(1) << (3 * level)); ++m) {
+      const int64_t* r = ranges + 4 * (base + m);
+      if (r[3] == r[2]) continue;
+      int x, y, z;
+      hipfmm::demorton3(uint32_t(m), level, x, y, z);
+      for (int ox = (x & 1) - 3; ox <= (x & 1) + 2; ++ox)
+        for (int oy = (y & 1) - 3; oy <= (y & 1) + 2; ++oy)
+          for (int oz = (z & 1) - 3; oz <= (z & 1) + 2; ++oz) {
+            if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) <= 1) continue;
+            const int sx = x - ox, sy = y - oy, sz = z - oz;
+            if (sx < 0 || sy < 0 || sz < 0 || sx >= nb || sy >= nb || sz >= nb) continue;
+            const int64_t* sr = ranges + 4 * (base + hipfmm::morton3(sx, sy, sz, level));
+            if (sr[1] != sr[0]) {
+              ++m2l_pairs;
+              if (level == L) ++leaf_pairs;
+            }
+          }
+    }
+  }
+  {
+    const int nb = 1 << L;
+    for (int64_t m = 0; m < d.leaves; ++m) {
+      const int64_t tc = lt[m].y - lt[m].x;
+      if (tc == 0) continue;
+      int x, y, z;
+      hipfmm::demorton3(uint32_t(m), L, x, y, z);
+      int64_t sc = 0;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int a = x + dx, b = y + dy, c = z + dz;
+            if (a < 0 || b < 0 || c < 0 || a >= nb || b >= nb || c >= nb) continue;
+            const int2 r = ls[hipfmm::morton3(a, b, c, L)];
+            sc += r.y - r.x;
+          }
+      p2p_pairs += uint64_t(tc * sc);
+    }
+  }
+  P->stats.m2l_interaction_count = m2l_pairs;
+  P->stats.m2l_leaf_interaction_count = leaf_pairs;
+  P->stats.p2p_interaction_count = p2p_pairs;
+  if constexpr (sizeof(S) == 4) {
+    if (L >= 2 && desc.m2l_class_count > 0) {
+      std::vector<double> mats(desc.m2l_matrices, desc.m2l_matrices + size_t(desc.m2l_class_count) * nn);
+      std::vector<double> lsd(desc.m2l_local_scaling, desc.m2l_local_scaling + size_t(L + 1) * n);
+      size_t bytes = 0;
+      hipfmm::prepare_mfma_operators(d, mats, desc.m2l_class_count, slot_of_class, lsd, P->persistent, bytes);
+      P->persistent_bytes += bytes;
+      P->stats.setup_h2d_bytes += bytes;
+      P->mfma_ready = true;
+    }
+  }
+}
+
+void free_state(hipfmm_plan* P) {
+  for (void* ptr : P->state) (void)hipFree(ptr);
+  P->state.clear();
+  P->state_bytes = 0;
+  P->d.kcap = 0;
+}
+
+template <class S>
+void ensure_capacity(hipfmm_plan* P, int K) {
+  DevicePlan& d = P->d;
+  if (K <= d.kcap) return;
+  free_state(P);
+  const int cap = std::min(hipfmm::kMaxBatch, std::max(K, 1));
+  const size_t cs = size_t(d.nodes) * d.n * cap;
+  auto alloc = [&](size_t count, size_t elem) {
+    return device_alloc(P, count * elem, P->state, P->state_bytes);
+  };
+  d.moments_in = alloc(size_t(std::max(d.ns, 1)) * 3 * cap, sizeof(S));
+  d.fields_out = alloc(size_t(std::max(d.nt, 1)) * 3 * cap, sizeof(S));
+  d.sm = alloc(size_t(std::max(d.ns, 1)) * 3 * cap, sizeof(S));
+  d.Hn = alloc(size_t(std::max(d.nt, 1)) * 3 * cap, sizeof(S));
+  d.M = alloc(cs, sizeof(S));
+  d.Lc = alloc(cs, sizeof(S));
+  HIPFMM_CHECK(hipMemset(d.M, 0, cs * sizeof(S)));
+  HIPFMM_CHECK(hipMemset(d.Lc, 0, cs * sizeof(S)));
+  if (sizeof(S) == 4 && P->mfma_ready) {
+    const size_t ms = size_t(d.nodes) * cap * d.apad;
+    d.mt_hi = alloc(ms, 2);
+    d.mt_lo = alloc(ms, 2);
+  }
+  d.kcap = cap;
+  P->stats.batch_capacity = uint64_t(cap);
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.0f;
+  }
+  return ms;
+}
+
+void finalize_timings(hipfmm_plan* P) {
+  if (!P->timings_pending) return;
+  HIPFMM_CHECK(hipEventSynchronize(P->ev[kEvL2P]));
+  if (P->last_host) HIPFMM_CHECK(hipEventSynchronize(P->ev[kEvD2H1]));
+  hipfmm_timings& t = P->timings;
+  hipEvent_t* e = P->ev;
+  t.h2d_seconds = 1e-3 * elapsed(e[kEvH2D0], e[kEvH2D1]);
+  t.p2m_seconds = 1e-3 * elapsed(e[kEvH2D1], e[kEvP2M]);
+  t.m2m_seconds = 1e-3 * elapsed(e[kEvP2M], e[kEvM2M]);
+  t.m2l_seconds = 1e-3 * elapsed(e[kEvM2M], e[kEvM2L]);
+  t.l2l_seconds = 1e-3 * elapsed(e[kEvM2L], e[kEvL2L]);
+  t.l2p_seconds = 1e-3 * elapsed(e[kEvL2L], e[kEvL2P]);
+  t.p2p_seconds = 1e-3 * elapsed(e[kEvNear0], e[kEvNear1]);
+  t.kernel_seconds = 1e-3 * elapsed(e[kEvH2D1], e[kEvL2P]);
+  t.d2h_seconds = P->last_host ? 1e-3 * elapsed(e[kEvD2H0], e[kEvD2H1]) : 0.0;
+  t.total_seconds = 1e-3 * elapsed(e[kEvH2D0], P->last_host ? e[kEvD2H1] : e[kEvL2P]);
+  t.m2l_leaf_seconds = P->d.L >= 2 ? 1e-3 * elapsed(e[kEvLeafM2L0], e[kEvLeafM2L1]) : 0.0;
+  P->timings_pending = false;
+}
+
+template <class S>
+void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, const int* self, int memspace,
+                   hipStream_t user) {
+  DevicePlan& d = P->d;
+  require(K >= 1 && K <= hipfmm::kMaxBatch, "batch must be between 1 and 64 moment states");
+  require(moments != nullptr || d.ns == 0, "moments pointer is null");
+  require(fields != nullptr || d.nt == 0, "fields pointer is null");
+  require(memspace == HIPFMM_HOST || memspace == HIPFMM_DEVICE, "unknown memory space");
+  // Identity map: fixed with the plan on first use (src/cuda_fmm.cu:3470-3486).
+  if (self != nullptr) {
+    if (!P->identity_initialised) {
+      for (int t = 0; t < d.nt; ++t)
+        require(self[t] >= -1 && self[t] < d.ns, "identity map contains an invalid index");
+      P->fixed_self.assign(self, self + d.nt);
+      if (d.nt) HIPFMM_CHECK(hipMemcpy(d.self, self, size_t(d.nt) * sizeof(int), hipMemcpyHostToDevice));
+      P->stats.setup_h2d_bytes += size_t(d.nt) * sizeof(int);
+      P->identity_initialised = true;
+    } else if (!std::equal(self, self + d.nt, P->fixed_self.begin())) {
+      throw hipfmm::invalid_argument("HipFull identity map changed; rebuild the static plan");
+    }
+  }
+  if (P->timings_pending) finalize_timings(P);
+  (void)hipGetLastError();
+  ensure_capacity<S>(P, K);
+  hipStream_t s = P->s_far;
+  hipEvent_t* e = P->ev;
+  const S* mu = nullptr;
+  S* out = nullptr;
+  S scale = S(1);
+  const size_t in_bytes = size_t(K) * d.ns * 3 * sizeof(S);
+  const size_t out_bytes = size_t(K) * d.nt * 3 * sizeof(S);
+  const bool host = memspace == HIPFMM_HOST;
+  if (host) {
+    if (P->pinned_in_bytes < in_bytes) {
+      if (P->pinned_in) (void)hipHostFree(P->pinned_in);
+      P->pinned_in = nullptr;
+      HIPFMM_CHECK(hipHostMalloc(&P->pinned_in, std::max<size_t>(in_bytes, 16)));
+      P->pinned_in_bytes = in_bytes;
+    }
+    if (P->pinned_out_bytes < out_bytes) {
+      if (P->pinned_out) (void)hipHostFree(P->pinned_out);
+      P->pinned_out = nullptr;
+      HIPFMM_CHECK(hipHostMalloc(&P->pinned_out, std::max<size_t>(out_bytes, 16)));
+      P->pinned_out_bytes = out_bytes;
+    }
+    if (in_bytes) std::memcpy(P->pinned_in, moments, in_bytes);
+    HIPFMM_CHECK(hipEventRecord(e[kEvH2D0], s));
+    if (in_bytes) HIPFMM_CHECK(hipMemcpyAsync(d.moments_in, P->pinned_in, in_bytes, hipMemcpyHostToDevice, s));
+    HIPFMM_CHECK(hipEventRecord(e[kEvH2D1], s));
+    mu = static_cast<const S*>(d.moments_in);
+    out = static_cast<S*>(d.fields_out);
+    P->stats.evaluation_h2d_bytes = in_bytes;
+    P->stats.evaluation_d2h_bytes = out_bytes;
+    P->stats.evaluation_h2d_calls += 1;
+    P->stats.evaluation_d2h_calls += 1;
+  } else {
+    HIPFMM_CHECK(hipEventRecord(e[kEvUser], user));
+    HIPFMM_CHECK(hipStreamWaitEvent(s, e[kEvUser], 0));
+    HIPFMM_CHECK(hipEventRecord(e[kEvH2D0], s));
+    HIPFMM_CHECK(hipEventRecord(e[kEvH2D1], s));
+    mu = static_cast<const S*>(moments);
+    out = static_cast<S*>(fields);
+    scale = S(P->device_moment_scale);
+    P->stats.evaluation_h2d_bytes = 0;
+    P->stats.evaluation_d2h_bytes = 0;
+  }
+  hipfmm::launch_gather<S>(d, mu, scale, K, s);
+  HIPFMM_CHECK(hipEventRecord(e[kEvGather], s));
+  HIPFMM_CHECK(hipStreamWaitEvent(P->s_near, e[kEvGather], 0));
+  HIPFMM_CHECK(hipEventRecord(e[kEvNear0], P->s_near));
+  hipfmm::launch_p2p<S>(d, K, P->s_near);
+  HIPFMM_CHECK(hipEventRecord(e[kEvNear1], P->s_near));
+  hipfmm::launch_p2m<S>(d, K, s);
+  HIPFMM_CHECK(hipEventRecord(e[kEvP2M], s));
+  for (int level = d.L - 1; level >= 2; --level) hipfmm::launch_m2m<S>(d, level, K, s);
+  // Levels 0 and 1 hold no list-2 work; keep their coefficients zero.
+  const size_t top = size_t(std::min<int64_t>(d.nodes, 9)) * d.n * K;
+  hipfmm::launch_zero<S>(static_cast<S*>(d.M), top, s);
+  hipfmm::launch_zero<S>(static_cast<S*>(d.Lc), d.L >= 2 ? top : size_t(d.nodes) * d.n * K, s);
+  HIPFMM_CHECK(hipEventRecord(e[kEvM2M], s));
+  const bool mfma = sizeof(S) == 4 && P->m2l_mode != HIPFMM_M2L_VALU && P->mfma_ready;
+  for (int level = 2; level <= d.L; ++level) {
+    if (mfma) hipfmm::launch_m2l_split(d, level, K, P->m2l_mode, s);
+    if (level == d.L) HIPFMM_CHECK(hipEventRecord(e[kEvLeafM2L0], s));
+    if (mfma)
+      hipfmm::launch_m2l_mfma(d, level, K, P->m2l_mode, s);
+    else
+      hipfmm::launch_m2l_valu<S>(d, level, K, s);
+    if (level == d.L) HIPFMM_CHECK(hipEventRecord(e[kEvLeafM2L1], s));
+  }
+  HIPFMM_CHECK(hipEventRecord(e[kEvM2L], s));
+  for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(d, level, K, s);
+  HIPFMM_CHECK(hipEventRecord(e[kEvL2L], s));
+  HIPFMM_CHECK(hipStreamWaitEvent(s, e[kEvNear1], 0));
+  hipfmm::launch_l2p_unsort<S>(d, out, K, s);
+  HIPFMM_CHECK(hipEventRecord(e[kEvL2P], s));
+  P->last_K = K;
+  P->last_host = host;
+  P->timings_pending = true;
+  if (host) {
+    HIPFMM_CHECK(hipEventRecord(e[kEvD2H0], s));
+    if (out_bytes) HIPFMM_CHECK(hipMemcpyAsync(P->pinned_out, d.fields_out, out_bytes, hipMemcpyDeviceToHost, s));
+    HIPFMM_CHECK(hipEventRecord(e[kEvD2H1], s));
+    HIPFMM_CHECK(hipStreamSynchronize(s));
+    if (out_bytes) std::memcpy(fields, P->pinned_out, out_bytes);
+    finalize_timings(P);
+  } else {
+    HIPFMM_CHECK(hipStreamWaitEvent(user, e[kEvL2P], 0));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hipfmm_last_error(void) { return g_last_error.c_str(); }
+
+int hipfmm_runtime_available(void) {
+  try {
+    const int count = gfx950_devices();
+    for (int i = 0; i < count; ++i)
+      if (device_is_gfx950(i)) return 1;
+  } catch (...) {
+  }
+  return 0;
+}
+
+int hipfmm_device_description(char* buffer, size_t length) {
+  return guarded([&] {
+    require(buffer != nullptr && length > 0, "buffer is empty");
+    std::string text;
+    const int count = gfx950_devices();
+    if (count == 0) {
+      text = "no HIP device is available";
+    } else {
+      int dev = 0;
+      HIPFMM_CHECK(hipGetDevice(&dev));
+      hipDeviceProp_t prop;
+      HIPFMM_CHECK(hipGetDeviceProperties(&prop, dev));
+      text = std::string(prop.name) + " (" + prop.gcnArchName + "), " + std::to_string(prop.multiProcessorCount) +
+             " CUs, " + std::to_string(prop.totalGlobalMem >> 20) + " MiB";
+    }
+    std::strncpy(buffer, text.c_str(), length - 1);
+    buffer[length - 1] = '\0';
+  });
+}
+
+int hipfmm_plan_create(const hipfmm_plan_desc* desc, hipfmm_plan** plan) {
+  return guarded([&] {
+    require(desc != nullptr && plan != nullptr, "descriptor and output must not be null");
+    *plan = nullptr;
+    require(desc->precision == 32 || desc->precision == 64, "precision must be 32 or 64");
+    require(desc->order >= 0 && desc->order <= hipfmm::kMaxOrder, "expansion order must be in [0, 10]");
+    require(desc->max_level >= 0 && desc->max_level <= 9, "max_level must be in [0, 9]");
+    require(desc->source_count >= 0 && desc->target_count >= 0, "point counts must be non-negative");
+    require((desc->source_count == 0 || desc->sorted_sources) && (desc->target_count == 0 || desc->sorted_targets),
+            "positions must not be null");
+    require(desc->node_ranges && desc->node_centres, "node ranges and centres must not be null");
+    require((desc->source_count == 0 || desc->source_permutation) &&
+                (desc->target_count == 0 || desc->target_permutation),
+            "permutations must not be null");
+    require(desc->m2l_class_count >= 0 && (desc->m2l_class_count == 0 || (desc->m2l_class_offsets && desc->m2l_matrices)),
+            "M2L classes are inconsistent");
+    require(desc->m2l_multipole_scaling && desc->m2l_local_scaling, "M2L scalings must not be null");
+    require(desc->max_level == 0 || (desc->m2m_operators && desc->l2l_operators), "shift operators must not be null");
+    require(desc->m2l_mode >= HIPFMM_M2L_VALU && desc->m2l_mode <= HIPFMM_M2L_MFMA_FP16X3, "unknown M2L mode");
+    require(desc->precision == 32 || desc->m2l_mode == HIPFMM_M2L_VALU, "MFMA M2L modes need an FP32 plan");
+    require(desc->m2l_mode != HIPFMM_M2L_MFMA_FP16X3, "the FP16 split M2L mode is not available in this build");
+    if (gfx950_devices() == 0) {
+      g_last_error = "HipFull is unavailable: no HIP device";
+      throw std::runtime_error(g_last_error);
+    }
+    auto P = std::make_unique<hipfmm_plan>();
+    HIPFMM_CHECK(hipGetDevice(&P->device));
+    HIPFMM_CHECK(hipStreamCreateWithFlags(&P->s_far, hipStreamNonBlocking));
+    HIPFMM_CHECK(hipStreamCreateWithFlags(&P->s_near, hipStreamNonBlocking));
+    for (auto& ev : P->ev) HIPFMM_CHECK(hipEventCreate(&ev));
+    P->device_moment_scale = desc->device_moment_scale;
+    P->m2l_mode = desc->m2l_mode;
+    try {
+      if (desc->precision == 32)
+        create_impl<float>(P.get(), *desc);
+      else
+        create_impl<double>(P.get(), *desc);
+    } catch (...) {
+      hipfmm_plan_destroy(P.release());
+      throw;
+    }
+    P->stats.plan_generation_count = 1;
+    P->stats.static_upload_count = 1;
+    *plan = P.release();
+  });
+}
+
+int hipfmm_plan_evaluate(hipfmm_plan* plan, const void* moments, void* fields, int batch,
+                         const int* sorted_self_indices, int memspace, void* stream) {
+  return guarded([&] {
+    require(plan != nullptr, "plan is null");
+    DeviceGuard guard(plan->device);
+    const hipStream_t user = static_cast<hipStream_t>(stream);
+    if (plan->d.precision == 32)
+      evaluate_impl<float>(plan, moments, fields, batch, sorted_self_indices, memspace, user);
+    else
+      evaluate_impl<double>(plan, moments, fields, batch, sorted_self_indices, memspace, user);
+  });
+}
+
+int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode) {
+  return guarded([&] {
+    require(plan != nullptr, "plan is null");
+    require(mode >= HIPFMM_M2L_VALU && mode <= HIPFMM_M2L_MFMA_FP16X3, "unknown M2L mode");
+    require(mode == HIPFMM_M2L_VALU || plan->d.precision == 32, "MFMA M2L modes need an FP32 plan");
+    require(mode != HIPFMM_M2L_MFMA_FP16X3, "the FP16 split M2L mode is not available in this build");
+    plan->m2l_mode = mode;
+  });
+}
+
+int hipfmm_plan_statistics(const hipfmm_plan* plan, hipfmm_stats* out) {
+  return guarded([&] {
+    require(plan != nullptr && out != nullptr, "plan and output must not be null");
+    *out = plan->stats;
+    out->persistent_device_bytes = plan->persistent_bytes + plan->state_bytes;
+  });
+}
+
+int hipfmm_plan_timings(const hipfmm_plan* plan, hipfmm_timings* out) {
+  return guarded([&] {
+    require(plan != nullptr && out != nullptr, "plan and output must not be null");
+    auto* P = const_cast<hipfmm_plan*>(plan);
+    DeviceGuard guard(P->device);
+    finalize_timings(P);
+    *out = P->timings;
+  });
+}
+
+int hipfmm_plan_coefficients(hipfmm_plan* plan, int which, int node, int state, double* out) {
+  return guarded([&] {
+    require(plan != nullptr && out != nullptr, "plan and output must not be null");
+    const DevicePlan& d = plan->d;
+    require(plan->last_K > 0, "no evaluation has run");
+    require(node >= 0 && node < d.nodes && state >= 0 && state < plan->last_K, "node or state out of range");
+    DeviceGuard guard(plan->device);
+    HIPFMM_CHECK(hipStreamSynchronize(plan->s_far));
+    const int K = plan->last_K;
+    const size_t elem = d.precision == 32 ? 4 : 8;
+    std::vector<unsigned char> buf(size_t(d.n) * K * elem);
+    const void* base = which == 0 ? d.M : d.Lc;
+    HIPFMM_CHECK(hipMemcpy(buf.data(), static_cast<const unsigned char*>(base) + size_t(node) * d.n * K * elem,
+                           buf.size(), hipMemcpyDeviceToHost));
+    for (int a = 0; a < d.n; ++a) {
+      const size_t i = size_t(a) * K + state;
+      out[a] = d.precision == 32 ? double(reinterpret_cast<const float*>(buf.data())[i])
+                                 : reinterpret_cast<const double*>(buf.data())[i];
+    }
+  });
+}
+
+void hipfmm_plan_destroy(hipfmm_plan* plan) {
+  if (plan == nullptr) return;
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(plan->device);
+  if (plan->s_far) (void)hipStreamSynchronize(plan->s_far);
+  if (plan->s_near) (void)hipStreamSynchronize(plan->s_near);
+  free_state(plan);
+  for (void* ptr : plan->persistent) (void)hipFree(ptr);
+  plan->persistent.clear();
+  if (plan->pinned_in) (void)hipHostFree(plan->pinned_in);
+  if (plan->pinned_out) (void)hipHostFree(plan->pinned_out);
+  for (auto& ev : plan->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (plan->s_far) (void)hipStreamDestroy(plan->s_far);
+  if (plan->s_near) (void)hipStreamDestroy(plan->s_near);
+  if (prev >= 0) (void)hipSetDevice(prev);
+  delete plan;
+}
+
+int hipfmm_direct_field(const double* targets, int nt, const double* sources, int ns, const double* moments,
+                        const int* self_indices, double* fields) {
+  return guarded([&] {
+    require(nt >= 0 && ns >= 0, "counts must be non-negative");
+    require((nt == 0 || (targets && fields)) && (ns == 0 || (sources && moments)), "arrays must not be null");
+    if (gfx950_devices() == 0) throw std::runtime_error("no HIP device is available");
+    double *t = nullptr, *s = nullptr, *m = nullptr, *f = nullptr;
+    int* id = nullptr;
+    struct Free {
+      std::vector<void*> ptrs;
+      ~Free() {
+        for (void* p : ptrs) (void)hipFree(p);
+      }
+    } cleanup;
+    auto dev = [&](auto*& ptr, const void* host, size_t bytes) {
+      HIPFMM_CHECK(hipMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(bytes, 16)));
+      cleanup.ptrs.push_back(ptr);
+      if (host && bytes) HIPFMM_CHECK(hipMemcpy(ptr, host, bytes, hipMemcpyHostToDevice));
+    };
+    dev(t, targets, size_t(nt) * 3 * sizeof(double));
+    dev(s, sources, size_t(ns) * 3 * sizeof(double));
+    dev(m, moments, size_t(ns) * 3 * sizeof(double));
+    dev(f, nullptr, size_t(nt) * 3 * sizeof(double));
+    if (self_indices) dev(id, self_indices, size_t(nt) * sizeof(int));
+    hipfmm::direct_field_fp64(t, nt, s, ns, m, id, f);
+    HIPFMM_CHECK(hipDeviceSynchronize());
+    if (nt) HIPFMM_CHECK(hipMemcpy(fields, f, size_t(nt) * 3 * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

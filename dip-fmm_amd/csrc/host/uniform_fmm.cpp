@@ -1,0 +1,614 @@
+#include "uniform_fmm.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+#include "../../../include/hipfmm.h"
+#include "operators.hpp"
+
+namespace cdfmm_hip {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+int64_t level_offset(int level) { return ((int64_t(1) << (3 * level)) - 1) / 7; }
+
+uint32_t morton3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+  uint32_t m = 0;
+  for (int b = 0; b < bits; ++b)
+    m |= (((x >> b) & 1u) << (3 * b)) | (((y >> b) & 1u) << (3 * b + 1)) | (((z >> b) & 1u) << (3 * b + 2));
+  return m;
+}
+
+void check(int status) {
+  if (status == HIPFMM_OK) return;
+  const std::string msg = hipfmm_last_error();
+  if (status == HIPFMM_ERROR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+const char* backend_name(ExecutionBackend b) {
+  switch (b) {
+    case ExecutionBackend::Auto: return "Auto";
+    case ExecutionBackend::CpuReference: return "CpuReference";
+    case ExecutionBackend::CpuStatic: return "CpuStatic";
+    case ExecutionBackend::CudaM2LP2P: return "CudaM2LP2P";
+    case ExecutionBackend::CudaFull: return "CudaFull";
+    case ExecutionBackend::HipFull: return "HipFull";
+  }
+  return "unknown";
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// UniformTree.  Geometry rules of src/uniform_tree.cpp:129-523: cubic root
+// around the combined bounding box (or user-fixed), leaf index by floor with
+// clamping and a 1e-12 boundary tolerance, Morton keys interleaving x,y,z
+// from the least significant bit, stable ordering of points sharing a leaf,
+// half-open node ranges with [count, count) for empty nodes.  The stable sort
+// is a counting sort over leaf keys, which yields the same permutation.
+// ---------------------------------------------------------------------------
+UniformTree::UniformTree(const std::vector<Vec3>& sources, const std::vector<Vec3>& targets,
+                         const UniformTreeOptions& options) {
+  if (options.max_level < 0) throw std::invalid_argument("UniformTreeOptions.max_level must be >= 0");
+  if (options.root_half_width.has_value() && options.root_half_width.value() <= 0.0)
+    throw std::invalid_argument("UniformTreeOptions.root_half_width must be positive");
+  if (options.max_level > 9) throw std::invalid_argument("UniformTreeOptions.max_level above 9 is not supported");
+  L_ = options.max_level;
+  const double inf = std::numeric_limits<double>::infinity();
+  Vec3 lo{inf, inf, inf}, hi{-inf, -inf, -inf};
+  bool any = false;
+  for (const auto* pts : {&sources, &targets})
+    for (const Vec3& p : *pts) {
+      any = true;
+      lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+      hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+    }
+  if (!any) lo = hi = Vec3{};
+  root_centre_ = options.root_centre.has_value()
+                     ? options.root_centre.value()
+                     : Vec3{(lo.x + hi.x) * 0.5, (lo.y + hi.y) * 0.5, (lo.z + hi.z) * 0.5};
+  if (options.root_half_width.has_value()) {
+    root_half_width_ = options.root_half_width.value();
+  } else if (!any) {
+    root_half_width_ = 1.0;
+  } else {
+    const Vec3& c = root_centre_;
+    root_half_width_ = std::max({hi.x - c.x, hi.y - c.y, hi.z - c.z, c.x - lo.x, c.y - lo.y, c.z - lo.z});
+  }
+  const double h = root_half_width_;
+  const double rmin[3] = {root_centre_.x - h, root_centre_.y - h, root_centre_.z - h};
+  const double rmax[3] = {root_centre_.x + h, root_centre_.y + h, root_centre_.z + h};
+  const int nb = 1 << L_;
+  const double cell = (2.0 * h) / static_cast<double>(nb);
+  constexpr double kTol = 1.0e-12;
+  auto leaf_key = [&](const Vec3& p) -> uint32_t {
+    const double v[3] = {p.x, p.y, p.z};
+    int idx[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) {
+      if (!(v[a] >= rmin[a] - kTol && v[a] <= rmax[a] + kTol))
+        throw std::invalid_argument("Point lies outside the requested root box");
+      if (L_ == 0) continue;
+      int i = static_cast<int>(std::floor((v[a] - rmin[a]) / cell));
+      idx[a] = std::clamp(i, 0, nb - 1);
+    }
+    return morton3(uint32_t(idx[0]), uint32_t(idx[1]), uint32_t(idx[2]), L_);
+  };
+  const int64_t leaves = int64_t(1) << (3 * L_);
+  const int64_t nodes = level_offset(L_ + 1);
+  ranges_.assign(static_cast<std::size_t>(4 * nodes), 0);
+  auto sort_population = [&](const std::vector<Vec3>& pts, std::vector<Vec3>& sorted, std::vector<int>& perm,
+                             std::vector<int>& inv, int column) {
+    const std::size_t count = pts.size();
+    std::vector<uint32_t> key(count);
+    std::vector<int64_t> start(static_cast<std::size_t>(leaves) + 1, 0);
+    for (std::size_t i = 0; i < count; ++i) {
+      key[i] = leaf_key(pts[i]);
+      ++start[key[i] + 1];
+    }
+    for (int64_t l = 0; l < leaves; ++l) start[l + 1] += start[l];
+    std::vector<int64_t> cursor(start.begin(), start.end() - 1);
+    sorted.resize(count);
+    perm.resize(count);
+    inv.resize(count);
+    for (std::size_t i = 0; i < count; ++i) {
+      const int64_t s = cursor[key[i]]++;
+      sorted[static_cast<std::size_t>(s)] = pts[i];
+      perm[static_cast<std::size_t>(s)] = static_cast<int>(i);
+      inv[i] = static_cast<int>(s);
+    }
+    const int64_t empty = static_cast<int64_t>(count);
+    const int64_t base = level_offset(L_);
+    for (int64_t l = 0; l < leaves; ++l) {
+      const bool occupied = start[l + 1] > start[l];
+      ranges_[4 * (base + l) + column] = occupied ? start[l] : empty;
+      ranges_[4 * (base + l) + column + 1] = occupied ? start[l + 1] : empty;
+    }
+    for (int level = L_ - 1; level >= 0; --level) {
+      const int64_t pb = level_offset(level), cb = level_offset(level + 1);
+      for (int64_t m = 0; m < (int64_t(1) << (3 * level)); ++m) {
+        int64_t b = empty, e = empty;
+        for (int c = 0; c < 8; ++c) {
+          const int64_t* r = &ranges_[4 * (cb + 8 * m + c) + column];
+          if (r[0] == r[1]) continue;
+          if (b == empty) b = r[0];
+          e = r[1];
+        }
+        ranges_[4 * (pb + m) + column] = b;
+        ranges_[4 * (pb + m) + column + 1] = e;
+      }
+    }
+  };
+  sort_population(sources, src_sorted_, src_perm_, src_inv_, 0);
+  sort_population(targets, tgt_sorted_, tgt_perm_, tgt_inv_, 2);
+}
+
+int UniformTree::node_level(int64_t node) const {
+  int level = 0;
+  while (level < L_ && level_offset(level + 1) <= node) ++level;
+  return level;
+}
+
+std::array<int, 3> UniformTree::node_coordinates(int64_t node) const {
+  const int level = node_level(node);
+  const uint64_t m = static_cast<uint64_t>(node - level_offset(level));
+  std::array<int, 3> c{0, 0, 0};
+  for (int b = 0; b < level; ++b)
+    for (int a = 0; a < 3; ++a) c[a] |= int((m >> (3 * b + a)) & 1u) << b;
+  return c;
+}
+
+// src/uniform_tree.cpp:250-257
+Vec3 UniformTree::node_centre(int64_t node) const {
+  const int level = node_level(node);
+  const auto c = node_coordinates(node);
+  const double n = static_cast<double>(1 << level);
+  const double h = root_half_width_;
+  return {root_centre_.x + ((static_cast<double>(c[0]) + 0.5) / n - 0.5) * 2.0 * h,
+          root_centre_.y + ((static_cast<double>(c[1]) + 0.5) / n - 0.5) * 2.0 * h,
+          root_centre_.z + ((static_cast<double>(c[2]) + 0.5) / n - 0.5) * 2.0 * h};
+}
+
+std::vector<int> UniformTree::list1(int64_t node) const {
+  const int level = node_level(node);
+  const auto c = node_coordinates(node);
+  const int nb = 1 << level;
+  std::vector<int> out;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = c[0] + dx, y = c[1] + dy, z = c[2] + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+        out.push_back(static_cast<int>(level_offset(level) + morton3(uint32_t(x), uint32_t(y), uint32_t(z), level)));
+      }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+// Offsets o = t - s with o_axis in [q-3, q+2] (q = coordinate parity) and
+// max|o| >= 2: the children of the parent's neighbours minus the near
+// neighbours (src/uniform_tree.cpp:486-515).
+std::vector<int> UniformTree::list2(int64_t node) const {
+  const int level = node_level(node);
+  std::vector<int> out;
+  if (level == 0) return out;
+  const auto c = node_coordinates(node);
+  const int nb = 1 << level;
+  for (int ox = (c[0] & 1) - 3; ox <= (c[0] & 1) + 2; ++ox)
+    for (int oy = (c[1] & 1) - 3; oy <= (c[1] & 1) + 2; ++oy)
+      for (int oz = (c[2] & 1) - 3; oz <= (c[2] & 1) + 2; ++oz) {
+        if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) <= 1) continue;
+        const int x = c[0] - ox, y = c[1] - oy, z = c[2] - oz;
+        if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+        out.push_back(static_cast<int>(level_offset(level) + morton3(uint32_t(x), uint32_t(y), uint32_t(z), level)));
+      }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Availability: cuda_full_available / cuda_device_description
+// (include/cdfmm/uniform_fmm.hpp:81-89).
+// ---------------------------------------------------------------------------
+bool hip_full_available() noexcept { return hipfmm_runtime_available() == 1; }
+
+std::string hip_device_description() {
+  char buf[256] = {0};
+  if (hipfmm_device_description(buf, sizeof(buf)) != HIPFMM_OK) return "no HIP device is available";
+  return buf;
+}
+
+// ---------------------------------------------------------------------------
+// UniformFmm
+// ---------------------------------------------------------------------------
+UniformFmm::UniformFmm(const std::vector<Vec3>& sources, const std::vector<Vec3>& targets,
+                       const UniformFmmOptions& options)
+    : tree_(sources, targets, options.tree),
+      basis_(std::max(options.expansion_order, 0)),
+      precision_(options.precision) {
+  if (options.expansion_order < 0) throw std::invalid_argument("UniformFmmOptions.expansion_order must be >= 0");
+  // Backend resolution (src/uniform_fmm.cpp:196-210).  Auto keeps the
+  // reference meaning (CpuStatic), which this library does not provide.
+  backend_ = options.backend == ExecutionBackend::Auto ? ExecutionBackend::CpuStatic : options.backend;
+  if (backend_ != ExecutionBackend::HipFull)
+    throw std::runtime_error(std::string(backend_name(backend_)) +
+                             " is provided by the reference library; this library implements HipFull only");
+  if (!hip_full_available()) throw std::runtime_error("HipFull is unavailable in this build");
+  // Fixed identity map (src/uniform_fmm.cpp:265-295).
+  if (options.fixed_target_source_indices.has_value()) {
+    const auto& ids = options.fixed_target_source_indices.value();
+    if (ids.size() != tree_.sorted_targets().size())
+      throw std::invalid_argument("fixed_target_source_indices must contain one entry per target");
+    for (const int s : ids)
+      if (s < -1 || s >= static_cast<int>(tree_.sorted_sources().size()))
+        throw std::invalid_argument("fixed_target_source_indices contains an invalid source index");
+    fixed_ids_ = ids;
+  }
+  m2l_mode_ = options.hip_m2l_mode;
+  build_plan(options);
+}
+
+UniformFmm::~UniformFmm() { hipfmm_plan_destroy(plan_); }
+
+UniformFmm::UniformFmm(UniformFmm&& other) noexcept
+    : tree_(std::move(other.tree_)),
+      basis_(std::move(other.basis_)),
+      precision_(other.precision_),
+      backend_(other.backend_),
+      m2l_mode_(other.m2l_mode_),
+      scale_(other.scale_),
+      fixed_ids_(std::move(other.fixed_ids_)),
+      plan_(other.plan_),
+      setup_seconds_(other.setup_seconds_),
+      last_timings_(other.last_timings_),
+      aggregate_timings_(other.aggregate_timings_) {
+  other.plan_ = nullptr;
+}
+
+UniformFmm& UniformFmm::operator=(UniformFmm&& other) noexcept {
+  if (this != &other) {
+    hipfmm_plan_destroy(plan_);
+    tree_ = std::move(other.tree_);
+    basis_ = std::move(other.basis_);
+    precision_ = other.precision_;
+    backend_ = other.backend_;
+    m2l_mode_ = other.m2l_mode_;
+    scale_ = other.scale_;
+    fixed_ids_ = std::move(other.fixed_ids_);
+    plan_ = other.plan_;
+    setup_seconds_ = other.setup_seconds_;
+    last_timings_ = other.last_timings_;
+    aggregate_timings_ = other.aggregate_timings_;
+    other.plan_ = nullptr;
+  }
+  return *this;
+}
+
+// Host plan construction (src/uniform_fmm.cpp:370-717 and :826-1013): the
+// FP32 operator frame, node geometry, M2L classes and level scalings, shift
+// operators and the identity map, packed into one descriptor and uploaded.
+void UniformFmm::build_plan(const UniformFmmOptions&) {
+  const auto start = Clock::now();
+  const int L = tree_.max_level(), n = basis_.size();
+  const int64_t nodes = tree_.node_count();
+  const bool fp32 = precision_ == StaticPrecision::Float32;
+  const Vec3 c = tree_.node_centre(0);
+  scale_ = fp32 ? 2.0 * tree_.root_half_width() : 1.0;
+  const double inv = 1.0 / scale_;
+  auto frame = [&](const Vec3& p, double* out) {
+    if (fp32) {
+      out[0] = (p.x - c.x) * inv;
+      out[1] = (p.y - c.y) * inv;
+      out[2] = (p.z - c.z) * inv;
+    } else {
+      out[0] = p.x;
+      out[1] = p.y;
+      out[2] = p.z;
+    }
+  };
+  const auto& ss = tree_.sorted_sources();
+  const auto& ts = tree_.sorted_targets();
+  std::vector<double> src(3 * ss.size()), tgt(3 * ts.size()), centres(3 * static_cast<std::size_t>(nodes));
+  for (std::size_t i = 0; i < ss.size(); ++i) frame(ss[i], &src[3 * i]);
+  for (std::size_t i = 0; i < ts.size(); ++i) frame(ts[i], &tgt[3 * i]);
+  for (int64_t k = 0; k < nodes; ++k) frame(tree_.node_centre(k), &centres[3 * static_cast<std::size_t>(k)]);
+  // M2L classes: every list-2 offset, matrices in FP64.
+  std::vector<std::array<int, 3>> offsets;
+  if (L >= 2) offsets = list2_offsets();
+  std::vector<int> class_offsets(3 * offsets.size());
+  std::vector<double> matrices(offsets.size() * static_cast<std::size_t>(n) * n);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (std::ptrdiff_t k = 0; k < static_cast<std::ptrdiff_t>(offsets.size()); ++k) {
+    const auto m = m2l_class_matrix(basis_, offsets[static_cast<std::size_t>(k)]);
+    std::copy(m.begin(), m.end(), matrices.begin() + k * static_cast<std::ptrdiff_t>(m.size()));
+  }
+  for (std::size_t k = 0; k < offsets.size(); ++k)
+    for (int a = 0; a < 3; ++a) class_offsets[3 * k + a] = offsets[k][a];
+  // Level scalings (src/uniform_fmm.cpp:529-566).
+  std::vector<double> ms(static_cast<std::size_t>(L + 1) * n), ls(ms.size());
+  std::vector<double> iwp(static_cast<std::size_t>(basis_.order() + 2), 1.0);
+  for (int level = 0; level <= L; ++level) {
+    const double physical = 2.0 * (tree_.root_half_width() / static_cast<double>(1 << level));
+    const double w = fp32 ? physical / scale_ : physical;
+    for (int d = 1; d <= basis_.order() + 1; ++d) iwp[d] = iwp[d - 1] / w;
+    for (int i = 0; i < n; ++i) {
+      ms[static_cast<std::size_t>(level) * n + i] = iwp[degree(basis_[i])];
+      ls[static_cast<std::size_t>(level) * n + i] = iwp[degree(basis_[i]) + 1];
+    }
+  }
+  // Shift operators per level and child class (src/uniform_fmm.cpp:457-500).
+  std::vector<double> m2m(static_cast<std::size_t>(L) * 8 * n * n), l2l(m2m.size());
+  for (int level = 1; level <= L; ++level) {
+    const double physical = tree_.root_half_width() / static_cast<double>(1 << level);
+    const double h = fp32 ? physical / scale_ : physical;
+    for (int cls = 0; cls < 8; ++cls) {
+      const std::array<double, 3> off{(cls & 1) ? h : -h, (cls & 2) ? h : -h, (cls & 4) ? h : -h};
+      const auto up = m2m_dense(basis_, {-off[0], -off[1], -off[2]});
+      const auto down = l2l_dense(basis_, off);
+      const std::size_t at = (static_cast<std::size_t>(level - 1) * 8 + cls) * n * n;
+      std::copy(up.begin(), up.end(), m2m.begin() + static_cast<std::ptrdiff_t>(at));
+      std::copy(down.begin(), down.end(), l2l.begin() + static_cast<std::ptrdiff_t>(at));
+    }
+  }
+  std::vector<int> fixed_self;
+  if (fixed_ids_.has_value()) fixed_self = sorted_self_indices(fixed_ids_.value());
+  hipfmm_plan_desc desc{};
+  desc.precision = fp32 ? 32 : 64;
+  desc.order = basis_.order();
+  desc.max_level = L;
+  desc.source_count = static_cast<int>(ss.size());
+  desc.target_count = static_cast<int>(ts.size());
+  desc.sorted_sources = src.data();
+  desc.sorted_targets = tgt.data();
+  desc.source_permutation = tree_.source_permutation().data();
+  desc.target_permutation = tree_.target_permutation().data();
+  desc.node_ranges = tree_.node_ranges().data();
+  desc.node_centres = centres.data();
+  desc.m2l_class_count = static_cast<int>(offsets.size());
+  desc.m2l_class_offsets = class_offsets.data();
+  desc.m2l_matrices = matrices.data();
+  desc.m2l_multipole_scaling = ms.data();
+  desc.m2l_local_scaling = ls.data();
+  desc.m2m_operators = m2m.data();
+  desc.l2l_operators = l2l.data();
+  desc.fixed_self_indices = fixed_self.empty() ? nullptr : fixed_self.data();
+  desc.device_moment_scale = fp32 ? 1.0 / (scale_ * scale_ * scale_) : 1.0;
+  desc.m2l_mode = static_cast<int>(m2l_mode_);
+  check(hipfmm_plan_create(&desc, &plan_));
+  setup_seconds_ = std::chrono::duration<double>(Clock::now() - start).count();
+}
+
+// src/uniform_fmm.cpp:1045-1063
+std::span<const int> UniformFmm::resolve_self_indices(std::span<const int> ids) const {
+  if (!fixed_ids_.has_value()) return ids;
+  const auto& fixed = fixed_ids_.value();
+  if (ids.empty()) return fixed;
+  if (!std::equal(ids.begin(), ids.end(), fixed.begin(), fixed.end()))
+    throw std::invalid_argument("fixed target-to-source identity map changed; rebuild the FMM plan");
+  return ids;
+}
+
+// src/uniform_fmm.cpp:1290-1305
+void UniformFmm::validate_identities(std::span<const int> ids, std::size_t moment_count) const {
+  if (!ids.empty() && ids.size() != tree_.sorted_targets().size())
+    throw std::invalid_argument("UniformFmm::evaluate identity map has incorrect length");
+  for (const int s : ids)
+    if (s < -1 || s >= static_cast<int>(moment_count))
+      throw std::invalid_argument("UniformFmm::evaluate identity map contains an invalid index");
+}
+
+// src/uniform_fmm.cpp:1028-1043
+std::vector<int> UniformFmm::sorted_self_indices(std::span<const int> ids) const {
+  std::vector<int> out(tree_.sorted_targets().size(), -1);
+  const auto& tperm = tree_.target_permutation();
+  const auto& sinv = tree_.source_inverse_permutation();
+  for (std::size_t t = 0; t < ids.size() && t < out.size(); ++t) {
+    const int s = ids[static_cast<std::size_t>(tperm[t])];
+    if (s >= 0) out[t] = sinv[static_cast<std::size_t>(s)];
+  }
+  return out;
+}
+
+void UniformFmm::record_timings() {
+  hipfmm_timings t{};
+  check(hipfmm_plan_timings(plan_, &t));
+  EvaluationTimings& l = last_timings_;
+  l.h2d = t.h2d_seconds;
+  l.p2m = t.p2m_seconds;
+  l.m2m = t.m2m_seconds;
+  l.m2l = t.m2l_seconds;
+  l.l2l = t.l2l_seconds;
+  l.l2p = t.l2p_seconds;
+  l.p2p = t.p2p_seconds;
+  l.kernel = t.kernel_seconds;
+  l.d2h = t.d2h_seconds;
+  l.total = t.total_seconds;
+  l.m2l_leaf = t.m2l_leaf_seconds;
+  l.evaluations = 1;
+  auto& a = aggregate_timings_;
+  a.h2d += t.h2d_seconds;
+  a.p2m += t.p2m_seconds;
+  a.m2m += t.m2m_seconds;
+  a.m2l += t.m2l_seconds;
+  a.l2l += t.l2l_seconds;
+  a.l2p += t.l2p_seconds;
+  a.p2p += t.p2p_seconds;
+  a.kernel += t.kernel_seconds;
+  a.d2h += t.d2h_seconds;
+  a.total += t.total_seconds;
+  a.m2l_leaf += t.m2l_leaf_seconds;
+  a.evaluations += 1;
+}
+
+EvaluationTimings UniformFmm::device_timings() {
+  record_timings();
+  return last_timings_;
+}
+
+void UniformFmm::evaluate_into_float32(std::span<const Vec3> moments, std::span<FloatPotentialField> results,
+                                       OutputFlags output, std::span<const int> ids) {
+  if (precision_ != StaticPrecision::Float32)
+    throw std::logic_error("evaluate_into_float32 requires an FP32 FMM plan");
+  const std::size_t nt = tree_.sorted_targets().size();
+  ids = resolve_self_indices(ids);
+  if (results.size() != nt)
+    throw std::invalid_argument("UniformFmm::evaluate_into_float32 requires one result per target");
+  validate_identities(ids, moments.size());
+  if (output != OutputFlags::Field) throw std::invalid_argument("HipFull currently supports field-only evaluation");
+  if (moments.size() != tree_.sorted_sources().size())
+    throw std::invalid_argument("full FP32 HipFull FMM dimensions are inconsistent");
+  const auto self = sorted_self_indices(ids);
+  std::vector<float> m(3 * moments.size());
+  for (std::size_t i = 0; i < moments.size(); ++i) {
+    const Vec3 v = moments[i];
+    m[3 * i] = static_cast<float>(v.x / scale_ / scale_ / scale_);
+    m[3 * i + 1] = static_cast<float>(v.y / scale_ / scale_ / scale_);
+    m[3 * i + 2] = static_cast<float>(v.z / scale_ / scale_ / scale_);
+  }
+  std::vector<float> f(3 * nt);
+  check(hipfmm_plan_evaluate(plan_, m.data(), f.data(), 1, self.data(), HIPFMM_HOST, nullptr));
+  for (std::size_t t = 0; t < nt; ++t) results[t] = {0.0F, {f[3 * t], f[3 * t + 1], f[3 * t + 2]}};
+  record_timings();
+}
+
+std::vector<FloatPotentialField> UniformFmm::evaluate_float32(std::span<const Vec3> moments, OutputFlags output,
+                                                              std::span<const int> ids) {
+  if (precision_ != StaticPrecision::Float32)
+    throw std::logic_error("evaluate_float32 requires an FP32 FMM plan");
+  std::vector<FloatPotentialField> r(tree_.sorted_targets().size());
+  evaluate_into_float32(moments, r, output, ids);
+  return r;
+}
+
+void UniformFmm::evaluate_into(std::span<const Vec3> moments, std::span<PotentialField> results, OutputFlags output,
+                               std::span<const int> ids) {
+  if (precision_ == StaticPrecision::Float32) {
+    std::vector<FloatPotentialField> f(results.size());
+    evaluate_into_float32(moments, f, output, ids);
+    for (std::size_t i = 0; i < results.size(); ++i)
+      results[i] = {double(f[i].phi), {double(f[i].H.x), double(f[i].H.y), double(f[i].H.z)}};
+    return;
+  }
+  const std::size_t nt = tree_.sorted_targets().size();
+  ids = resolve_self_indices(ids);
+  if (results.size() != nt) throw std::invalid_argument("UniformFmm::evaluate_into requires one result per target");
+  validate_identities(ids, moments.size());
+  if (output != OutputFlags::Field) throw std::invalid_argument("HipFull currently supports field-only evaluation");
+  if (moments.size() != tree_.sorted_sources().size())
+    throw std::invalid_argument("full FP64 HipFull FMM dimensions are inconsistent");
+  const auto self = sorted_self_indices(ids);
+  std::vector<double> m(3 * moments.size());
+  for (std::size_t i = 0; i < moments.size(); ++i) {
+    m[3 * i] = moments[i].x;
+    m[3 * i + 1] = moments[i].y;
+    m[3 * i + 2] = moments[i].z;
+  }
+  std::vector<double> f(3 * nt);
+  check(hipfmm_plan_evaluate(plan_, m.data(), f.data(), 1, self.data(), HIPFMM_HOST, nullptr));
+  for (std::size_t t = 0; t < nt; ++t) results[t] = {0.0, {f[3 * t], f[3 * t + 1], f[3 * t + 2]}};
+  record_timings();
+}
+
+std::vector<PotentialField> UniformFmm::evaluate(std::span<const Vec3> moments, OutputFlags output,
+                                                 std::span<const int> ids) {
+  std::vector<PotentialField> r(tree_.sorted_targets().size());
+  evaluate_into(moments, r, output, ids);
+  return r;
+}
+
+std::vector<PotentialField> UniformFmm::evaluate_float64(std::span<const Vec3> moments, OutputFlags output,
+                                                         std::span<const int> ids) {
+  if (precision_ != StaticPrecision::Float64)
+    throw std::logic_error("evaluate_float64 requires an FP64 FMM plan");
+  return evaluate(moments, output, ids);
+}
+
+void UniformFmm::evaluate_batch_into(std::span<const Vec3> moments, int batch, void* fields,
+                                     std::span<const int> ids) {
+  const std::size_t ns = tree_.sorted_sources().size();
+  if (batch < 1 || moments.size() != ns * static_cast<std::size_t>(batch))
+    throw std::invalid_argument("batched moments must hold batch x source_count vectors");
+  ids = resolve_self_indices(ids);
+  validate_identities(ids, ns);
+  const auto self = sorted_self_indices(ids);
+  if (precision_ == StaticPrecision::Float32) {
+    std::vector<float> m(3 * moments.size());
+    for (std::size_t i = 0; i < moments.size(); ++i) {
+      m[3 * i] = static_cast<float>(moments[i].x / scale_ / scale_ / scale_);
+      m[3 * i + 1] = static_cast<float>(moments[i].y / scale_ / scale_ / scale_);
+      m[3 * i + 2] = static_cast<float>(moments[i].z / scale_ / scale_ / scale_);
+    }
+    check(hipfmm_plan_evaluate(plan_, m.data(), fields, batch, self.data(), HIPFMM_HOST, nullptr));
+  } else {
+    std::vector<double> m(3 * moments.size());
+    for (std::size_t i = 0; i < moments.size(); ++i) {
+      m[3 * i] = moments[i].x;
+      m[3 * i + 1] = moments[i].y;
+      m[3 * i + 2] = moments[i].z;
+    }
+    check(hipfmm_plan_evaluate(plan_, m.data(), fields, batch, self.data(), HIPFMM_HOST, nullptr));
+  }
+  record_timings();
+}
+
+void UniformFmm::evaluate_device(const void* moments, void* fields, int batch, void* stream) {
+  const int* self = nullptr;
+  std::vector<int> s;
+  if (fixed_ids_.has_value()) {
+    s = sorted_self_indices(fixed_ids_.value());
+    self = s.data();
+  }
+  check(hipfmm_plan_evaluate(plan_, moments, fields, batch, self, HIPFMM_DEVICE, stream));
+}
+
+void UniformFmm::set_m2l_mode(HipM2LMode mode) {
+  check(hipfmm_plan_set_m2l_mode(plan_, static_cast<int>(mode)));
+  m2l_mode_ = mode;
+}
+
+HipPlanStatistics UniformFmm::hip_plan_statistics() const {
+  hipfmm_stats s{};
+  check(hipfmm_plan_statistics(plan_, &s));
+  HipPlanStatistics o;
+  o.scalar_bytes = s.scalar_bytes;
+  o.m2m_unique_matrix_count = s.m2m_unique_matrix_count;
+  o.m2m_matrix_bytes = s.m2m_matrix_bytes;
+  o.m2l_unique_matrix_count = s.m2l_unique_matrix_count;
+  o.m2l_matrix_bytes = s.m2l_matrix_bytes;
+  o.m2l_interaction_count = s.m2l_interaction_count;
+  o.l2l_unique_matrix_count = s.l2l_unique_matrix_count;
+  o.l2l_matrix_bytes = s.l2l_matrix_bytes;
+  o.p2p_interaction_count = s.p2p_interaction_count;
+  o.setup_h2d_bytes = s.setup_h2d_bytes;
+  o.evaluation_h2d_bytes = s.evaluation_h2d_bytes;
+  o.evaluation_d2h_bytes = s.evaluation_d2h_bytes;
+  o.evaluation_h2d_calls = s.evaluation_h2d_calls;
+  o.evaluation_d2h_calls = s.evaluation_d2h_calls;
+  o.persistent_device_bytes = s.persistent_device_bytes;
+  o.plan_generation_count = s.plan_generation_count;
+  o.static_upload_count = s.static_upload_count;
+  o.batch_capacity = s.batch_capacity;
+  o.m2l_leaf_interaction_count = s.m2l_leaf_interaction_count;
+  o.setup_seconds = setup_seconds_;
+  return o;
+}
+
+std::vector<double> UniformFmm::multipole(int node, int state) const {
+  std::vector<double> out(static_cast<std::size_t>(basis_.size()));
+  check(hipfmm_plan_coefficients(plan_, 0, node, state, out.data()));
+  return out;
+}
+
+std::vector<double> UniformFmm::local(int node, int state) const {
+  std::vector<double> out(static_cast<std::size_t>(basis_.size()));
+  check(hipfmm_plan_coefficients(plan_, 1, node, state, out.data()));
+  return out;
+}
+
+}  // namespace cdfmm_hip

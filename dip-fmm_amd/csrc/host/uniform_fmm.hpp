@@ -1,0 +1,209 @@
+// HipFull host front-end: the reference's UniformFmm interface
+// (include/cdfmm/uniform_fmm.hpp:22-442) for the device-resident backend.
+//
+// The host builds the uniform octree (Morton order, permutations, node
+// ranges, list1/list2), the identity map and the FP64 translation operators
+// once, and uploads them through the C-ABI of include/hipfmm.h.  Every
+// evaluate then runs entirely on the GPU.  Names, argument meaning and
+// exception types follow the reference so that a caller of
+// cdfmm::UniformFmm(..., backend = CudaFull) can switch to HipFull.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <span>
+#include <string>
+#include <vector>
+
+#include "basis.hpp"
+
+struct hipfmm_plan;
+
+namespace cdfmm_hip {
+
+struct Vec3 {
+  double x{0.0}, y{0.0}, z{0.0};
+};
+struct FloatVec3 {
+  float x{0.0F}, y{0.0F}, z{0.0F};
+};
+struct PotentialField {
+  double phi{0.0};
+  Vec3 H{};
+};
+struct FloatPotentialField {
+  float phi{0.0F};
+  FloatVec3 H{};
+};
+
+// include/cdfmm/output_flags.hpp:12-17
+enum class OutputFlags : unsigned { None = 0u, Potential = 1u, Field = 2u, Both = 3u };
+// include/cdfmm/precision.hpp:11-14
+enum class StaticPrecision { Float32, Float64 };
+// include/cdfmm/uniform_fmm.hpp:22-31, plus HipFull.
+enum class ExecutionBackend {
+  Auto,
+  CpuReference,
+  CpuStatic,
+  CudaM2LP2P,
+  CudaPartial = CudaM2LP2P,
+  CudaFull,
+  CudaM2L = CudaM2LP2P,
+  CudaM2LStaticP2P = CudaM2LP2P,
+  HipFull = 5
+};
+// M2L arithmetic of the HipFull device plan (enum hipfmm_m2l_mode).
+enum class HipM2LMode { Valu = 0, MfmaBf16x3 = 1, MfmaFp16x3 = 2 };
+
+// include/cdfmm/uniform_tree.hpp:28-39
+struct UniformTreeOptions {
+  int max_level{1};
+  bool include_empty_nodes{true};
+  bool cubic_root_box{true};
+  std::optional<Vec3> root_centre{};
+  std::optional<double> root_half_width{};
+};
+
+// include/cdfmm/uniform_fmm.hpp:98-135 (point-dipole fields), plus the
+// HipFull M2L mode.
+struct UniformFmmOptions {
+  StaticPrecision precision{StaticPrecision::Float32};
+  int expansion_order{4};
+  UniformTreeOptions tree{};
+  ExecutionBackend backend{ExecutionBackend::Auto};
+  std::optional<std::vector<int>> fixed_target_source_indices{};
+  HipM2LMode hip_m2l_mode{HipM2LMode::Valu};
+};
+
+// Complete uniform octree with level-wise Morton ordering; the same geometry
+// as the reference's UniformTree (src/uniform_tree.cpp:129-523).
+class UniformTree {
+ public:
+  UniformTree(const std::vector<Vec3>& sources, const std::vector<Vec3>& targets, const UniformTreeOptions& options);
+
+  int max_level() const { return L_; }
+  const Vec3& root_centre() const { return root_centre_; }
+  double root_half_width() const { return root_half_width_; }
+  int64_t node_count() const { return static_cast<int64_t>(ranges_.size() / 4); }
+  // Per node: source begin, source end, target begin, target end.
+  const std::vector<int64_t>& node_ranges() const { return ranges_; }
+  Vec3 node_centre(int64_t node) const;
+  int node_level(int64_t node) const;
+  std::array<int, 3> node_coordinates(int64_t node) const;
+  const std::vector<Vec3>& sorted_sources() const { return src_sorted_; }
+  const std::vector<Vec3>& sorted_targets() const { return tgt_sorted_; }
+  const std::vector<int>& source_permutation() const { return src_perm_; }
+  const std::vector<int>& source_inverse_permutation() const { return src_inv_; }
+  const std::vector<int>& target_permutation() const { return tgt_perm_; }
+  const std::vector<int>& target_inverse_permutation() const { return tgt_inv_; }
+  // Near neighbours (clipped 3x3x3, including the node) and well-separated
+  // children of the parent's neighbours, sorted by node index.  Both are
+  // enumerated from integer box offsets exactly as the device kernels do.
+  std::vector<int> list1(int64_t node) const;
+  std::vector<int> list2(int64_t node) const;
+
+ private:
+  int L_{0};
+  Vec3 root_centre_{};
+  double root_half_width_{0.0};
+  std::vector<int64_t> ranges_;
+  std::vector<Vec3> src_sorted_, tgt_sorted_;
+  std::vector<int> src_perm_, src_inv_, tgt_perm_, tgt_inv_;
+};
+
+// Most recent evaluation, seconds (include/cdfmm/timings.hpp:15-104 subset).
+struct EvaluationTimings {
+  double h2d{0.0}, p2m{0.0}, m2m{0.0}, m2l{0.0}, l2l{0.0}, l2p{0.0}, p2p{0.0}, kernel{0.0}, d2h{0.0};
+  double total{0.0};
+  double m2l_leaf{0.0};  // leaf-level M2L product kernel
+  std::uint64_t evaluations{0};
+};
+
+// Device statistics (include/cdfmm/timings.hpp:182-222 subset).
+struct HipPlanStatistics {
+  std::size_t scalar_bytes{0}, m2m_unique_matrix_count{0}, m2m_matrix_bytes{0}, m2l_unique_matrix_count{0},
+      m2l_matrix_bytes{0}, m2l_interaction_count{0}, l2l_unique_matrix_count{0}, l2l_matrix_bytes{0},
+      p2p_interaction_count{0}, setup_h2d_bytes{0}, evaluation_h2d_bytes{0}, evaluation_d2h_bytes{0},
+      evaluation_h2d_calls{0}, evaluation_d2h_calls{0}, persistent_device_bytes{0}, plan_generation_count{0},
+      static_upload_count{0}, batch_capacity{0}, m2l_leaf_interaction_count{0};
+  double setup_seconds{0.0};  // host plan construction + upload
+};
+
+bool hip_full_available() noexcept;
+std::string hip_device_description();
+
+class UniformFmm {
+ public:
+  UniformFmm(const std::vector<Vec3>& sources, const std::vector<Vec3>& targets,
+             const UniformFmmOptions& options = {});
+  ~UniformFmm();
+  UniformFmm(UniformFmm&&) noexcept;
+  UniformFmm& operator=(UniformFmm&&) noexcept;
+  UniformFmm(const UniformFmm&) = delete;
+  UniformFmm& operator=(const UniformFmm&) = delete;
+
+  // include/cdfmm/uniform_fmm.hpp:217-288
+  std::vector<PotentialField> evaluate(std::span<const Vec3> moments, OutputFlags output = OutputFlags::Field,
+                                       std::span<const int> target_source_indices = {});
+  void evaluate_into(std::span<const Vec3> moments, std::span<PotentialField> results,
+                     OutputFlags output = OutputFlags::Field, std::span<const int> target_source_indices = {});
+  std::vector<FloatPotentialField> evaluate_float32(std::span<const Vec3> moments,
+                                                    OutputFlags output = OutputFlags::Field,
+                                                    std::span<const int> target_source_indices = {});
+  void evaluate_into_float32(std::span<const Vec3> moments, std::span<FloatPotentialField> results,
+                             OutputFlags output = OutputFlags::Field,
+                             std::span<const int> target_source_indices = {});
+  std::vector<PotentialField> evaluate_float64(std::span<const Vec3> moments,
+                                               OutputFlags output = OutputFlags::Field,
+                                               std::span<const int> target_source_indices = {});
+
+  // K moment states in one device pass.  moments: K x N_s user-order vectors;
+  // fields: K x N_t x 3 scalars of the plan precision (float or double).
+  void evaluate_batch_into(std::span<const Vec3> moments, int batch, void* fields,
+                           std::span<const int> target_source_indices = {});
+  // Device-resident operands (K x N x 3 scalars of the plan precision);
+  // moments in physical units; ordered after `stream` (hipStream_t or null).
+  void evaluate_device(const void* moments, void* fields, int batch, void* stream);
+  // Stage times of the most recent evaluate (waits for a device-resident
+  // evaluate to finish).
+  EvaluationTimings device_timings();
+
+  void set_m2l_mode(HipM2LMode mode);
+  HipM2LMode m2l_mode() const { return m2l_mode_; }
+
+  const UniformTree& tree() const { return tree_; }
+  const Basis& basis() const { return basis_; }
+  ExecutionBackend backend() const { return backend_; }
+  StaticPrecision precision() const noexcept { return precision_; }
+  // Root width W of the FP32 operator frame (1 for FP64 plans).
+  double coordinate_scale() const { return scale_; }
+  const EvaluationTimings& last_timings() const { return last_timings_; }
+  const EvaluationTimings& aggregate_timings() const { return aggregate_timings_; }
+  void reset_timings() { aggregate_timings_ = {}; }
+  HipPlanStatistics hip_plan_statistics() const;
+  // Node coefficients of the last evaluate (state 0 unless given).
+  std::vector<double> multipole(int node, int state = 0) const;
+  std::vector<double> local(int node, int state = 0) const;
+
+ private:
+  void build_plan(const UniformFmmOptions& options);
+  std::span<const int> resolve_self_indices(std::span<const int> ids) const;
+  void validate_identities(std::span<const int> ids, std::size_t moment_count) const;
+  std::vector<int> sorted_self_indices(std::span<const int> ids) const;
+  void record_timings();
+
+  UniformTree tree_;
+  Basis basis_;
+  StaticPrecision precision_;
+  ExecutionBackend backend_{ExecutionBackend::HipFull};
+  HipM2LMode m2l_mode_{HipM2LMode::Valu};
+  double scale_{1.0};
+  std::optional<std::vector<int>> fixed_ids_{};
+  hipfmm_plan* plan_{nullptr};
+  double setup_seconds_{0.0};
+  EvaluationTimings last_timings_{}, aggregate_timings_{};
+};
+
+}  // namespace cdfmm_hip

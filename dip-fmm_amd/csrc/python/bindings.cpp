@@ -1,0 +1,310 @@
+// Python extension `_cdfmm_hip` (imported through cdfmm_hip.py): the reference's `cdfmm` Python interface
+// (python/bindings.cpp:391-422, :584-774) for the HipFull backend.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../host/operators.hpp"
+#include "../host/uniform_fmm.hpp"
+
+namespace py = pybind11;
+using namespace cdfmm_hip;
+
+namespace {
+
+using DoubleArray = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+// python/bindings.cpp:59-77
+std::vector<Vec3> parse_vec3_array(const py::handle& input, const std::string& name) {
+  const DoubleArray a = py::cast<DoubleArray>(input);
+  const auto info = a.request();
+  if (info.ndim != 2 || info.shape[1] != 3) throw std::invalid_argument(name + " must have shape (n, 3)");
+  const auto v = a.unchecked<2>();
+  std::vector<Vec3> out(static_cast<std::size_t>(info.shape[0]));
+  for (py::ssize_t i = 0; i < info.shape[0]; ++i) out[static_cast<std::size_t>(i)] = {v(i, 0), v(i, 1), v(i, 2)};
+  return out;
+}
+
+OutputFlags parse_output(const std::string& output) {
+  if (output == "field") return OutputFlags::Field;
+  if (output == "potential") return OutputFlags::Potential;
+  if (output == "both") return OutputFlags::Both;
+  throw std::invalid_argument("output must be 'field', 'potential', or 'both'");
+}
+
+template <class T>
+py::dict fields_to_dict(const std::vector<T>& r) {
+  using S = decltype(r[0].phi);
+  py::array_t<S> phi(static_cast<py::ssize_t>(r.size()));
+  py::array_t<S> H({static_cast<py::ssize_t>(r.size()), py::ssize_t(3)});
+  auto p = phi.template mutable_unchecked<1>();
+  auto h = H.template mutable_unchecked<2>();
+  for (py::ssize_t i = 0; i < static_cast<py::ssize_t>(r.size()); ++i) {
+    p(i) = r[static_cast<std::size_t>(i)].phi;
+    h(i, 0) = r[static_cast<std::size_t>(i)].H.x;
+    h(i, 1) = r[static_cast<std::size_t>(i)].H.y;
+    h(i, 2) = r[static_cast<std::size_t>(i)].H.z;
+  }
+  py::dict d;
+  d["phi"] = phi;
+  d["H"] = H;
+  return d;
+}
+
+py::array_t<int> int_array(const std::vector<int>& v) {
+  py::array_t<int> a(static_cast<py::ssize_t>(v.size()));
+  std::copy(v.begin(), v.end(), a.mutable_data());
+  return a;
+}
+
+py::dict timings_dict(const EvaluationTimings& t) {
+  py::dict d;
+  d["h2d"] = t.h2d;
+  d["p2m"] = t.p2m;
+  d["m2m"] = t.m2m;
+  d["m2l"] = t.m2l;
+  d["l2l"] = t.l2l;
+  d["l2p"] = t.l2p;
+  d["p2p"] = t.p2p;
+  d["kernel"] = t.kernel;
+  d["d2h"] = t.d2h;
+  d["total"] = t.total;
+  d["m2l_leaf"] = t.m2l_leaf;
+  d["evaluations"] = t.evaluations;
+  return d;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_cdfmm_hip, m) {
+  m.doc() = "HipFull: MI355X-native static-geometry dipole FMM evaluate behind the cdfmm UniformFmm interface";
+
+  py::class_<Vec3>(m, "Vec3")
+      .def(py::init<double, double, double>())
+      .def_readwrite("x", &Vec3::x)
+      .def_readwrite("y", &Vec3::y)
+      .def_readwrite("z", &Vec3::z);
+
+  py::enum_<StaticPrecision>(m, "StaticPrecision")
+      .value("FLOAT32", StaticPrecision::Float32)
+      .value("FLOAT64", StaticPrecision::Float64);
+
+  py::enum_<ExecutionBackend>(m, "ExecutionBackend")
+      .value("AUTO", ExecutionBackend::Auto)
+      .value("CPU_REFERENCE", ExecutionBackend::CpuReference)
+      .value("CPU_STATIC", ExecutionBackend::CpuStatic)
+      .value("CUDA_M2L_P2P", ExecutionBackend::CudaM2LP2P)
+      .value("CUDA_PARTIAL", ExecutionBackend::CudaPartial)
+      .value("CUDA_FULL", ExecutionBackend::CudaFull)
+      .value("CUDA_M2L", ExecutionBackend::CudaM2L)
+      .value("CUDA_M2L_STATIC_P2P", ExecutionBackend::CudaM2LStaticP2P)
+      .value("HIP_FULL", ExecutionBackend::HipFull);
+
+  py::enum_<HipM2LMode>(m, "HipM2LMode")
+      .value("VALU", HipM2LMode::Valu)
+      .value("MFMA_BF16X3", HipM2LMode::MfmaBf16x3)
+      .value("MFMA_FP16X3", HipM2LMode::MfmaFp16x3);
+
+  m.def("hip_full_available", &hip_full_available);
+  // Host operator builders, exposed for the parity tests of the plan.
+  m.def("_laplace_derivatives", [](int order, std::array<double, 3> r) {
+    const auto v = laplace_derivatives(order, r);
+    return py::array_t<double>(static_cast<py::ssize_t>(v.size()), v.data());
+  });
+  m.def("_m2l_class_matrix", [](int order, std::array<int, 3> offset) {
+    const Basis b(order);
+    const auto v = m2l_class_matrix(b, offset);
+    const py::ssize_t n = b.size();
+    py::array_t<double> a({n, n});  // returned as T[beta, alpha]
+    auto w = a.mutable_unchecked<2>();
+    for (py::ssize_t al = 0; al < n; ++al)
+      for (py::ssize_t be = 0; be < n; ++be) w(be, al) = v[static_cast<std::size_t>(be + n * al)];
+    return a;
+  });
+  m.def("hip_device_description", &hip_device_description);
+
+  py::class_<UniformTreeOptions>(m, "UniformTreeOptions")
+      .def(py::init<>())
+      .def_readwrite("max_level", &UniformTreeOptions::max_level)
+      .def_readwrite("include_empty_nodes", &UniformTreeOptions::include_empty_nodes)
+      .def_readwrite("cubic_root_box", &UniformTreeOptions::cubic_root_box)
+      .def_readwrite("root_centre", &UniformTreeOptions::root_centre)
+      .def_readwrite("root_half_width", &UniformTreeOptions::root_half_width);
+
+  py::class_<UniformTree>(m, "UniformTree")
+      .def(py::init([](py::object s, py::object t, const UniformTreeOptions& o) {
+             return UniformTree(parse_vec3_array(s, "source_positions"), parse_vec3_array(t, "target_positions"), o);
+           }),
+           py::arg("source_positions"), py::arg("target_positions"), py::arg("options"))
+      .def_property_readonly("max_level", &UniformTree::max_level)
+      .def_property_readonly("root_centre", &UniformTree::root_centre)
+      .def_property_readonly("root_half_width", &UniformTree::root_half_width)
+      .def_property_readonly("node_count", &UniformTree::node_count)
+      .def_property_readonly("source_permutation", [](const UniformTree& t) { return int_array(t.source_permutation()); })
+      .def_property_readonly("source_inverse_permutation",
+                             [](const UniformTree& t) { return int_array(t.source_inverse_permutation()); })
+      .def_property_readonly("target_permutation", [](const UniformTree& t) { return int_array(t.target_permutation()); })
+      .def_property_readonly("target_inverse_permutation",
+                             [](const UniformTree& t) { return int_array(t.target_inverse_permutation()); })
+      .def_property_readonly("node_ranges",
+                             [](const UniformTree& t) {
+                               py::array_t<int64_t> a({t.node_count(), int64_t(4)});
+                               std::copy(t.node_ranges().begin(), t.node_ranges().end(), a.mutable_data());
+                               return a;
+                             })
+      .def("node_centre", [](const UniformTree& t, int64_t node) {
+        const Vec3 c = t.node_centre(node);
+        return py::make_tuple(c.x, c.y, c.z);
+      })
+      .def("list1", [](const UniformTree& t, int64_t node) { return int_array(t.list1(node)); })
+      .def("list2", [](const UniformTree& t, int64_t node) { return int_array(t.list2(node)); });
+
+  py::class_<UniformFmmOptions>(m, "UniformFmmOptions")
+      .def(py::init<>())
+      .def_readwrite("precision", &UniformFmmOptions::precision)
+      .def_readwrite("expansion_order", &UniformFmmOptions::expansion_order)
+      .def_readwrite("tree", &UniformFmmOptions::tree)
+      .def_readwrite("backend", &UniformFmmOptions::backend)
+      .def_readwrite("fixed_target_source_indices", &UniformFmmOptions::fixed_target_source_indices)
+      .def_readwrite("hip_m2l_mode", &UniformFmmOptions::hip_m2l_mode);
+
+  py::class_<UniformFmm>(m, "UniformFmm")
+      .def(py::init([](py::object s, py::object t, const UniformFmmOptions& o) {
+             const auto src = parse_vec3_array(s, "source_positions");
+             const auto tgt = parse_vec3_array(t, "target_positions");
+             py::gil_scoped_release release;
+             return std::make_unique<UniformFmm>(src, tgt, o);
+           }),
+           py::arg("source_positions"), py::arg("target_positions"), py::arg("options") = UniformFmmOptions{})
+      .def(
+          "evaluate",
+          [](UniformFmm& f, py::object moments, const std::string& output, py::object ids) {
+            const auto m = parse_vec3_array(moments, "dipole_moments");
+            std::vector<int> id;
+            if (!ids.is_none()) id = py::cast<std::vector<int>>(ids);
+            const OutputFlags flags = parse_output(output);
+            if (f.precision() == StaticPrecision::Float32) {
+              std::vector<FloatPotentialField> r;
+              {
+                py::gil_scoped_release release;
+                r = f.evaluate_float32(m, flags, id);
+              }
+              return fields_to_dict(r);
+            }
+            std::vector<PotentialField> r;
+            {
+              py::gil_scoped_release release;
+              r = f.evaluate(m, flags, id);
+            }
+            return fields_to_dict(r);
+          },
+          py::arg("dipole_moments"), py::arg("output") = "field", py::arg("target_source_indices") = py::none())
+      .def(
+          "evaluate_batch",
+          [](UniformFmm& f, py::object moments, py::object ids) -> py::object {
+            const DoubleArray a = py::cast<DoubleArray>(moments);
+            const auto info = a.request();
+            if (info.ndim != 3 || info.shape[2] != 3)
+              throw std::invalid_argument("dipole_moments must have shape (batch, n, 3)");
+            const int K = static_cast<int>(info.shape[0]);
+            const py::ssize_t ns = info.shape[1];
+            std::vector<Vec3> m(static_cast<std::size_t>(K * ns));
+            const double* p = a.data();
+            for (std::size_t i = 0; i < m.size(); ++i) m[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+            std::vector<int> id;
+            if (!ids.is_none()) id = py::cast<std::vector<int>>(ids);
+            const py::ssize_t nt = static_cast<py::ssize_t>(f.tree().sorted_targets().size());
+            if (f.precision() == StaticPrecision::Float32) {
+              py::array_t<float> out({py::ssize_t(K), nt, py::ssize_t(3)});
+              float* o = out.mutable_data();
+              {
+                py::gil_scoped_release release;
+                f.evaluate_batch_into(m, K, o, id);
+              }
+              return out;
+            }
+            py::array_t<double> out({py::ssize_t(K), nt, py::ssize_t(3)});
+            double* o = out.mutable_data();
+            {
+              py::gil_scoped_release release;
+              f.evaluate_batch_into(m, K, o, id);
+            }
+            return out;
+          },
+          py::arg("dipole_moments"), py::arg("target_source_indices") = py::none(),
+          "Evaluate K moment states (batch, n, 3) in one device pass; returns (batch, n_targets, 3).")
+      .def(
+          "evaluate_device",
+          [](UniformFmm& f, std::uintptr_t moments, std::uintptr_t fields, int batch, std::uintptr_t stream) {
+            py::gil_scoped_release release;
+            f.evaluate_device(reinterpret_cast<const void*>(moments), reinterpret_cast<void*>(fields), batch,
+                              reinterpret_cast<void*>(stream));
+          },
+          py::arg("moments_ptr"), py::arg("fields_ptr"), py::arg("batch"), py::arg("stream") = 0,
+          "Device-resident evaluate: pointers to (batch, n, 3) moments and (batch, n_targets, 3) fields of the "
+          "plan precision; ordered after `stream`.")
+      .def("device_timings", [](UniformFmm& f) {
+        EvaluationTimings t;
+        {
+          py::gil_scoped_release release;
+          t = f.device_timings();
+        }
+        return timings_dict(t);
+      })
+      .def("set_m2l_mode", &UniformFmm::set_m2l_mode)
+      .def_property_readonly("m2l_mode", &UniformFmm::m2l_mode)
+      .def_property_readonly("tree", &UniformFmm::tree, py::return_value_policy::reference_internal)
+      .def_property_readonly("expansion_order", [](const UniformFmm& f) { return f.basis().order(); })
+      .def_property_readonly("coefficient_count", [](const UniformFmm& f) { return f.basis().size(); })
+      .def_property_readonly("backend", &UniformFmm::backend)
+      .def_property_readonly("precision", &UniformFmm::precision)
+      .def_property_readonly("coordinate_scale", &UniformFmm::coordinate_scale)
+      .def_property_readonly("last_timings", [](const UniformFmm& f) { return timings_dict(f.last_timings()); })
+      .def_property_readonly("aggregate_timings", [](const UniformFmm& f) { return timings_dict(f.aggregate_timings()); })
+      .def("reset_timings", &UniformFmm::reset_timings)
+      .def_property_readonly("hip_plan_statistics",
+                             [](const UniformFmm& f) {
+                               const auto s = f.hip_plan_statistics();
+                               py::dict d;
+                               d["scalar_bytes"] = s.scalar_bytes;
+                               d["m2m_unique_matrix_count"] = s.m2m_unique_matrix_count;
+                               d["m2m_matrix_bytes"] = s.m2m_matrix_bytes;
+                               d["m2l_unique_matrix_count"] = s.m2l_unique_matrix_count;
+                               d["m2l_matrix_bytes"] = s.m2l_matrix_bytes;
+                               d["m2l_interaction_count"] = s.m2l_interaction_count;
+                               d["l2l_unique_matrix_count"] = s.l2l_unique_matrix_count;
+                               d["l2l_matrix_bytes"] = s.l2l_matrix_bytes;
+                               d["p2p_interaction_count"] = s.p2p_interaction_count;
+                               d["setup_h2d_bytes"] = s.setup_h2d_bytes;
+                               d["evaluation_h2d_bytes"] = s.evaluation_h2d_bytes;
+                               d["evaluation_d2h_bytes"] = s.evaluation_d2h_bytes;
+                               d["evaluation_h2d_calls"] = s.evaluation_h2d_calls;
+                               d["evaluation_d2h_calls"] = s.evaluation_d2h_calls;
+                               d["persistent_device_bytes"] = s.persistent_device_bytes;
+                               d["plan_generation_count"] = s.plan_generation_count;
+                               d["static_upload_count"] = s.static_upload_count;
+                               d["batch_capacity"] = s.batch_capacity;
+                               d["m2l_leaf_interaction_count"] = s.m2l_leaf_interaction_count;
+                               d["setup_seconds"] = s.setup_seconds;
+                               return d;
+                             })
+      .def(
+          "multipole",
+          [](const UniformFmm& f, int node, int state) {
+            const auto v = f.multipole(node, state);
+            return py::array_t<double>(static_cast<py::ssize_t>(v.size()), v.data());
+          },
+          py::arg("node_index"), py::arg("state") = 0)
+      .def(
+          "local",
+          [](const UniformFmm& f, int node, int state) {
+            const auto v = f.local(node, state);
+            return py::array_t<double>(static_cast<py::ssize_t>(v.size()), v.data());
+          },
+          py::arg("node_index"), py::arg("state") = 0);
+}

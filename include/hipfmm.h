@@ -1,0 +1,208 @@
+/*
+ * hipfmm.h -- C-ABI of the HipFull device plan (one MI355X / gfx950 GPU).
+ *
+ * This is the boundary a host FMM front-end binds to replace the reference's
+ * device plan: `cdfmm::CudaFullPlan` (src/cuda_fmm_plan.hpp:95-114, built by
+ * `UniformFmm::build_cuda_full_plan`, src/uniform_fmm.cpp:826-1013, and
+ * driven by `UniformFmm::evaluate_into[_float32]`, src/uniform_fmm.cpp:1102-
+ * 1136 and :1307-1349).  The host keeps building the Morton octree,
+ * permutations, identity map, translation operators and M2L class matrices
+ * once; `hipfmm_plan_create` uploads them a single time.  Every evaluate then
+ * runs P2M, M2M, M2L, L2L, L2P, list-1 P2P and the unsort to user target
+ * order on the device.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; no C++ or framework types.
+ *  - Every function returns a status (HIPFMM_OK == 0).  On failure a
+ *    thread-local message is available from hipfmm_last_error().  The C++
+ *    front-end maps HIPFMM_ERROR_INVALID_ARGUMENT to std::invalid_argument and
+ *    every other failure to std::runtime_error, as the reference does
+ *    (src/cuda_fmm.cu:32-54, :3462-3486).
+ *  - Coefficient order is the reference's total-degree multi-index order
+ *    (include/cdfmm/multi_index.hpp:62-71); n = (p+1)(p+2)(p+3)/6.
+ *  - Nodes are stored level by level, Morton order within a level
+ *    (src/uniform_tree.cpp:100-113); a complete tree has (8^(L+1)-1)/7 nodes.
+ *  - One plan is not re-entrant (include/cdfmm/uniform_fmm.hpp:222-229);
+ *    separate plans may be used concurrently.
+ */
+#ifndef HIPFMM_H
+#define HIPFMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hipfmm_plan hipfmm_plan;
+
+enum hipfmm_status {
+  HIPFMM_OK = 0,
+  HIPFMM_ERROR_RUNTIME = 1,          /* device or runtime failure */
+  HIPFMM_ERROR_INVALID_ARGUMENT = 2, /* dimensions, identity map, options */
+  HIPFMM_ERROR_UNAVAILABLE = 3       /* no usable HIP device */
+};
+
+enum hipfmm_memspace { HIPFMM_HOST = 0, HIPFMM_DEVICE = 1 };
+
+/* M2L arithmetic.  VALU is the plain path (FP32 plans: FP32 FMA; FP64 plans:
+ * FP64 FMA).  The MFMA modes exist for FP32 plans only: operands are split
+ * into two 16-bit halves (hi + lo) and three products are accumulated in FP32
+ * on the matrix cores. */
+enum hipfmm_m2l_mode {
+  HIPFMM_M2L_VALU = 0,
+  HIPFMM_M2L_MFMA_BF16X3 = 1,
+  HIPFMM_M2L_MFMA_FP16X3 = 2
+};
+
+/* Plan descriptor.  All arrays are host memory owned by the caller; the
+ * create call copies what it needs, so they may be freed afterwards. */
+typedef struct hipfmm_plan_desc {
+  int precision;    /* 32 or 64 (StaticPrecision, include/cdfmm/precision.hpp:11) */
+  int order;        /* expansion order p >= 0 */
+  int max_level;    /* leaf level L >= 0 */
+  int source_count;
+  int target_count;
+  /* Morton-sorted positions, 3 doubles each, in the operator frame: root-
+   * width normalised for FP32 plans (src/uniform_fmm.cpp:379-417), physical
+   * for FP64 plans. */
+  const double* sorted_sources;
+  const double* sorted_targets;
+  const int* source_permutation; /* sorted -> user index (uniform_tree.hpp:128) */
+  const int* target_permutation;
+  /* Per node: source begin, source end, target begin, target end (half-open,
+   * [count, count) when empty; src/uniform_tree.cpp:359-438). */
+  const int64_t* node_ranges;
+  /* Per node centre, 3 doubles, operator frame. */
+  const double* node_centres;
+  /* M2L class matrices (src/static_operators.cpp:159-184): `m2l_class_count`
+   * integer offsets (target minus source box coordinates, 3 ints each) and a
+   * column-major n x n matrix per class, T[beta + n*alpha] = D_{alpha+beta}G.
+   * Every offset the device meets must be present (all 316 list-2 offsets of
+   * a tree with L >= 2 cover every case). */
+  int m2l_class_count;
+  const int* m2l_class_offsets;
+  const double* m2l_matrices;
+  /* Level scalings, (L+1) x n each (src/uniform_fmm.cpp:529-566). */
+  const double* m2l_multipole_scaling;
+  const double* m2l_local_scaling;
+  /* Shift operators for levels 1..L and child class c = (ix&1)|(iy&1)<<1|
+   * (iz&1)<<2, dense row-major n x n (out x in): M2M maps a child multipole
+   * into its parent, L2L a parent local into its child
+   * (src/static_operators.cpp:257-298).  Size L*8*n*n each. */
+  const double* m2m_operators;
+  const double* l2l_operators;
+  /* Optional fixed identity map: sorted target -> sorted source or -1
+   * (src/uniform_fmm.cpp:265-295, :1028-1043).  NULL when not fixed. */
+  const int* fixed_self_indices;
+  /* Factor applied to moments supplied in device memory before use; FP32
+   * plans pass 1/W^3 for root width W so the caller may hand over physical
+   * moments (src/uniform_fmm.cpp:1315-1323).  Host moments are assumed to be
+   * scaled already and use factor 1. */
+  double device_moment_scale;
+  int m2l_mode; /* enum hipfmm_m2l_mode */
+} hipfmm_plan_desc;
+
+/* Host/device traffic and storage; mirrors CudaPlanStatistics
+ * (include/cdfmm/timings.hpp:182-222). */
+typedef struct hipfmm_stats {
+  uint64_t scalar_bytes;
+  uint64_t m2m_unique_matrix_count;
+  uint64_t m2m_matrix_bytes;
+  uint64_t m2l_unique_matrix_count;
+  uint64_t m2l_matrix_bytes;
+  uint64_t m2l_interaction_count;
+  uint64_t l2l_unique_matrix_count;
+  uint64_t l2l_matrix_bytes;
+  uint64_t p2p_interaction_count;
+  uint64_t setup_h2d_bytes;
+  uint64_t evaluation_h2d_bytes;  /* of the most recent host-memory evaluate */
+  uint64_t evaluation_d2h_bytes;
+  uint64_t evaluation_h2d_calls;  /* cumulative */
+  uint64_t evaluation_d2h_calls;
+  uint64_t persistent_device_bytes;
+  uint64_t plan_generation_count;
+  uint64_t static_upload_count;
+  uint64_t batch_capacity;        /* moment states the state buffers hold */
+  uint64_t m2l_leaf_interaction_count; /* non-empty list-2 pairs at the leaf level */
+} hipfmm_stats;
+
+/* Device-stream stage times of the most recent evaluate, seconds; mirrors
+ * CudaEvaluationTimings (include/cdfmm/timings.hpp:225-241). */
+typedef struct hipfmm_timings {
+  double h2d_seconds;
+  double p2m_seconds;   /* moment gather + P2M */
+  double m2m_seconds;
+  double m2l_seconds;   /* operand split + M2L */
+  double l2l_seconds;
+  double l2p_seconds;   /* L2P + combine + unsort */
+  double p2p_seconds;   /* near-field stream */
+  double kernel_seconds;/* first device stage to last */
+  double d2h_seconds;
+  double total_seconds;
+  double m2l_leaf_seconds; /* the leaf-level M2L product kernel alone */
+} hipfmm_timings;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* hipfmm_last_error(void);
+
+/* 1 when a HIP device with gfx950 code objects is usable, else 0.
+ * Replaces cuda_full_available() (include/cdfmm/uniform_fmm.hpp:81-86). */
+int hipfmm_runtime_available(void);
+
+/* Device name, architecture and memory.  Replaces cuda_device_description()
+ * (include/cdfmm/uniform_fmm.hpp:88-89). */
+int hipfmm_device_description(char* buffer, size_t length);
+
+/* Upload the static plan once.  Replaces CudaFullPlan::CudaFullPlan
+ * (src/cuda_fmm_plan.hpp:97-98; src/cuda_fmm.cu:2714-3155). */
+int hipfmm_plan_create(const hipfmm_plan_desc* desc, hipfmm_plan** plan);
+
+/* Evaluate `batch` moment states.  Replaces CudaFullPlan::evaluate
+ * (src/cuda_fmm_plan.hpp:104-108; src/cuda_fmm.cu:3238-3655) and adds the
+ * batch dimension and device-resident operands.
+ *   moments: batch x source_count x 3 scalars in user source order
+ *   fields:  batch x target_count x 3 scalars in user target order
+ *   scalar type is float for FP32 plans and double for FP64 plans;
+ *   memspace says where both live.  Host moments must already carry the
+ *   FP32 frame scaling; device moments are multiplied by
+ *   desc.device_moment_scale.
+ *   sorted_self_indices: host array of target_count entries (sorted target
+ *   -> sorted source, or -1) or NULL.  The first map seen is fixed with the
+ *   plan; a later different map fails with HIPFMM_ERROR_INVALID_ARGUMENT
+ *   (src/cuda_fmm.cu:3470-3486).  NULL reuses the fixed map (none = -1).
+ *   stream: hipStream_t to order device-memory work after, or NULL.
+ * Host-memory calls return when the fields are in host memory; device-memory
+ * calls return once the work is enqueued on the plan's streams and ordered
+ * before any later work on `stream`. */
+int hipfmm_plan_evaluate(hipfmm_plan* plan, const void* moments, void* fields, int batch,
+                         const int* sorted_self_indices, int memspace, void* stream);
+
+/* Select the M2L arithmetic of later evaluates (enum hipfmm_m2l_mode). */
+int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode);
+
+/* Statistics / timings.  Replace CudaFullPlan::statistics() / timings(). */
+int hipfmm_plan_statistics(const hipfmm_plan* plan, hipfmm_stats* out);
+int hipfmm_plan_timings(const hipfmm_plan* plan, hipfmm_timings* out);
+
+/* Copy the node-major coefficients of the last evaluate for one node and
+ * moment state (which: 0 multipole, 1 local), n doubles.  Replaces the host
+ * multipole()/local() accessors (src/uniform_fmm.cpp:1538-1612). */
+int hipfmm_plan_coefficients(hipfmm_plan* plan, int which, int node, int state, double* out);
+
+/* Free every device and pinned buffer.  Idempotent; NULL is accepted. */
+void hipfmm_plan_destroy(hipfmm_plan* plan);
+
+/* FP64 O(N_t N_s) direct field with self exclusion on the device: the GPU
+ * form of direct_p2p_reference (src/validation.cpp:71-111; model kernel
+ * src/cuda_fmm.cu:518-569).  Host arrays: targets nt x 3, sources and moments
+ * ns x 3, self_indices nt (or NULL), fields nt x 3 out. */
+int hipfmm_direct_field(const double* targets, int nt, const double* sources, int ns,
+                        const double* moments, const int* self_indices, double* fields);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPFMM_H */

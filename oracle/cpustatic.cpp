@@ -1,0 +1,1156 @@
+// Test infrastructure only -- the oracle.  Never linked into the product.
+//
+// CPU restatement of the reference CpuStatic uniform FMM (FP32 and FP64) and
+// of its FP64 direct sum.  Expressions keep the reference's operand order so
+// that, compiled with the reference's own options (oracle/Makefile), results
+// agree with the compiled reference bit for bit or to the last few ulps.
+// Operator values that the reference stores in FP64 and then quantises are
+// computed here in FP64 and rounded once, which yields the same FP32 values
+// without the FP64 staging copy.  Parity of each piece is pinned in
+// tests/test_oracle.py (see the header cpustatic.h).
+
+#include "cpustatic.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <numbers>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace oracle {
+namespace {
+
+thread_local std::string g_last_error;
+
+struct V3 {
+  double x{0.0}, y{0.0}, z{0.0};
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+struct F3 {
+  float x{0.0F}, y{0.0F}, z{0.0F};
+};
+
+using MI = std::array<int, 3>;
+
+// ---------------------------------------------------------------------------
+// Basis: include/cdfmm/multi_index.hpp:59-129.  Degree-major, then ax, then
+// ay; an O(1) lookup table replaces the reference's linear index() search
+// (same result).
+// ---------------------------------------------------------------------------
+struct Basis {
+  int p{0};
+  std::vector<MI> a;
+  std::vector<int> lut;
+  explicit Basis(int order) : p(order) {
+    for (int d = 0; d <= p; ++d)
+      for (int ax = 0; ax <= d; ++ax)
+        for (int ay = 0; ay <= d - ax; ++ay) a.push_back({ax, ay, d - ax - ay});
+    lut.assign(static_cast<std::size_t>((p + 1) * (p + 1) * (p + 1)), -1);
+    for (int i = 0; i < size(); ++i) lut[key(a[i])] = i;
+  }
+  int size() const { return static_cast<int>(a.size()); }
+  std::size_t key(const MI& m) const {
+    return static_cast<std::size_t>((m[0] * (p + 1) + m[1]) * (p + 1) + m[2]);
+  }
+  int index(const MI& m) const {  // multi_index.hpp:81-90
+    if (m[0] < 0 || m[1] < 0 || m[2] < 0 || m[0] + m[1] + m[2] > p)
+      throw std::out_of_range("multi-index not found");
+    return lut[key(m)];
+  }
+};
+
+inline int degree(const MI& m) { return m[0] + m[1] + m[2]; }
+inline bool leq(const MI& a, const MI& b) { return a[0] <= b[0] && a[1] <= b[1] && a[2] <= b[2]; }
+
+// multi_index.hpp:94-106
+double factorial(int n) {
+  double v = 1.0;
+  for (int i = 2; i <= n; ++i) v *= i;
+  return v;
+}
+double multi_factorial(const MI& a) { return factorial(a[0]) * factorial(a[1]) * factorial(a[2]); }
+
+// multi_index.hpp:112-115
+double monomial_over_factorial(const V3& r, const MI& a) {
+  return std::pow(r.x, a[0]) * std::pow(r.y, a[1]) * std::pow(r.z, a[2]) / multi_factorial(a);
+}
+
+// ---------------------------------------------------------------------------
+// Taylor jets: src/taylor_jet.cpp:45-148, src/laplace_derivatives.cpp:21-42.
+// For each alpha the reference scans the basis in order and keeps the gamma
+// with gamma <= alpha; those (gamma, alpha-gamma) index pairs are precomputed
+// here once, in the same order, so every sum is accumulated identically.
+// ---------------------------------------------------------------------------
+struct Jets {
+  Basis b;
+  std::vector<int> off, jg, kd;  // per alpha: pairs (gamma index, (alpha-gamma) index)
+  explicit Jets(int order) : b(order) {
+    off.push_back(0);
+    for (int i = 0; i < b.size(); ++i) {
+      for (int j = 0; j < b.size(); ++j) {
+        if (!leq(b.a[j], b.a[i])) continue;
+        jg.push_back(j);
+        kd.push_back(b.index({b.a[i][0] - b.a[j][0], b.a[i][1] - b.a[j][1], b.a[i][2] - b.a[j][2]}));
+      }
+      off.push_back(static_cast<int>(jg.size()));
+    }
+  }
+  // taylor_jet.cpp:45-66
+  std::vector<double> mul(const std::vector<double>& x, const std::vector<double>& y) const {
+    std::vector<double> o(x.size());
+    for (int i = 0; i < b.size(); ++i) {
+      double sum = 0.0;
+      for (int e = off[i]; e < off[i + 1]; ++e) sum += x[jg[e]] * y[kd[e]];
+      o[i] = sum;
+    }
+    return o;
+  }
+  static std::vector<double> add(const std::vector<double>& x, const std::vector<double>& y) {
+    std::vector<double> o(x.size());
+    for (std::size_t i = 0; i < x.size(); ++i) o[i] = x[i] + y[i];
+    return o;
+  }
+  // taylor_jet.cpp:68-148.  w_gamma = (y*y)_gamma is cached: every y it
+  // reads has lower degree than the alpha that first needs it, so its value
+  // equals the reference's recomputation.
+  std::vector<double> invsqrt(const std::vector<double>& c) const {
+    const int n = b.size();
+    std::vector<double> y(c.size(), 0.0), w(c.size(), 0.0);
+    std::vector<char> have_w(c.size(), 0);
+    const double z0 = c[0];
+    if (z0 <= 0.0) throw std::invalid_argument("TaylorJet::invsqrt requires positive constant coefficient");
+    y[0] = 1.0 / std::sqrt(z0);
+    for (int i = 1; i < n; ++i) {
+      double sum_lower = 0.0;
+      for (int e = off[i]; e < off[i + 1]; ++e) {
+        const int gamma = jg[e];
+        if (gamma == i) continue;
+        if (!have_w[gamma]) {
+          double wg = 0.0;
+          for (int f = off[gamma]; f < off[gamma + 1]; ++f) wg += y[jg[f]] * y[kd[f]];
+          w[gamma] = wg;
+          have_w[gamma] = 1;
+        }
+        sum_lower += w[gamma] * c[kd[e]];
+      }
+      const double w_alpha = -sum_lower / z0;
+      double nonlinear = 0.0;
+      for (int e = off[i]; e < off[i + 1]; ++e) {
+        if (jg[e] == 0 || kd[e] == 0) continue;
+        nonlinear += y[jg[e]] * y[kd[e]];
+      }
+      y[i] = (w_alpha - nonlinear) / (2.0 * y[0]);
+    }
+    return y;
+  }
+  // laplace_derivatives.cpp:21-42: raw D_alpha G(r), G = 1/(4 pi |r|).
+  std::vector<double> laplace_raw(const V3& r) const {
+    const int n = b.size();
+    auto coordinate = [&](int axis, double v) {
+      std::vector<double> t(n, 0.0);
+      t[0] = v;
+      MI e{0, 0, 0};
+      e[axis] = 1;
+      t[b.index(e)] = 1.0;
+      return t;
+    };
+    const auto x = coordinate(0, r.x), y = coordinate(1, r.y), z = coordinate(2, r.z);
+    const auto rho2 = add(add(mul(x, x), mul(y, y)), mul(z, z));
+    const auto inv_r = invsqrt(rho2);
+    std::vector<double> k(n, 0.0);
+    k[0] = 1.0 / (4.0 * std::numbers::pi);
+    const auto G = mul(inv_r, k);
+    std::vector<double> out(n);
+    for (int i = 0; i < n; ++i) out[i] = G[i] * multi_factorial(b.a[i]);
+    return out;
+  }
+};
+
+// static_operators.cpp:159-184: column-major T[beta + n*alpha] = D_{alpha+beta}G(R).
+std::vector<double> m2l_matrix(const Basis& basis, const Jets& jets, const V3& R) {
+  const int n = basis.size();
+  const auto D = jets.laplace_raw(R);
+  std::vector<double> m(static_cast<std::size_t>(n) * n);
+  for (int a = 0; a < n; ++a)
+    for (int bb = 0; bb < n; ++bb) {
+      const MI g{basis.a[a][0] + basis.a[bb][0], basis.a[a][1] + basis.a[bb][1],
+                 basis.a[a][2] + basis.a[bb][2]};
+      m[static_cast<std::size_t>(bb) + static_cast<std::size_t>(n) * a] =
+          D[static_cast<std::size_t>(jets.b.index(g))];
+    }
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Uniform tree: src/uniform_tree.cpp:71-523.
+// ---------------------------------------------------------------------------
+std::uint64_t morton_encode(int ix, int iy, int iz) {  // uniform_tree.cpp:71-85
+  std::uint64_t m = 0;
+  for (int bit = 0; bit < 21; ++bit) {
+    m |= ((static_cast<std::uint64_t>(ix) >> bit) & 1ULL) << (3 * bit);
+    m |= ((static_cast<std::uint64_t>(iy) >> bit) & 1ULL) << (3 * bit + 1);
+    m |= ((static_cast<std::uint64_t>(iz) >> bit) & 1ULL) << (3 * bit + 2);
+  }
+  return m;
+}
+int level_offset(int level) {  // uniform_tree.cpp:100-108
+  std::size_t num = 1;
+  for (int i = 0; i < level; ++i) num *= 8;
+  return static_cast<int>((num - 1) / 7);
+}
+int node_index(int level, int ix, int iy, int iz) {  // uniform_tree.cpp:110-113
+  return level_offset(level) + static_cast<int>(morton_encode(ix, iy, iz));
+}
+
+struct Node {
+  int level{0}, ix{0}, iy{0}, iz{0}, parent{-1};
+  std::array<int, 8> children{{-1, -1, -1, -1, -1, -1, -1, -1}};
+  V3 centre{};
+  double half_width{0.0};
+  std::size_t sb{0}, se{0}, tb{0}, te{0};
+  std::vector<int> list1, list2;
+  std::size_t scount() const { return se - sb; }
+  std::size_t tcount() const { return te - tb; }
+};
+
+struct Tree {
+  int L{0};
+  V3 root_centre{};
+  double root_hw{0.0};
+  std::vector<Node> nodes;
+  std::vector<V3> src, tgt;  // Morton-sorted
+  std::vector<int> src_perm, src_inv, src_leaf, tgt_perm, tgt_inv, tgt_leaf;
+  std::vector<int> occ_src, occ_tgt;
+
+  void build(const std::vector<V3>& sources, const std::vector<V3>& targets, int max_level,
+             bool has_root, V3 centre, double half_width) {
+    constexpr double kTol = 1.0e-12;  // uniform_tree.cpp:22
+    if (max_level < 0) throw std::invalid_argument("UniformTreeOptions.max_level must be >= 0");
+    L = max_level;
+    // Root cube: uniform_tree.cpp:143-196.
+    const double inf = std::numeric_limits<double>::infinity();
+    V3 mn{inf, inf, inf}, mx{-inf, -inf, -inf};
+    bool has_points = false;
+    for (const auto* pts : {&sources, &targets})
+      for (const V3& p : *pts) {
+        has_points = true;
+        mn.x = std::min(mn.x, p.x);
+        mn.y = std::min(mn.y, p.y);
+        mn.z = std::min(mn.z, p.z);
+        mx.x = std::max(mx.x, p.x);
+        mx.y = std::max(mx.y, p.y);
+        mx.z = std::max(mx.z, p.z);
+      }
+    if (!has_points) {
+      mn = {0.0, 0.0, 0.0};
+      mx = {0.0, 0.0, 0.0};
+    }
+    root_centre = has_root ? centre : (mn + mx) * 0.5;
+    if (half_width > 0.0) {
+      root_hw = half_width;
+    } else {
+      const V3 dmax = mx - root_centre, dmin = root_centre - mn;
+      root_hw = std::max({dmax.x, dmax.y, dmax.z, dmin.x, dmin.y, dmin.z});
+      if (!has_points) root_hw = 1.0;
+    }
+    const V3 rmin = root_centre - V3{root_hw, root_hw, root_hw};
+    const V3 rmax = root_centre + V3{root_hw, root_hw, root_hw};
+    // Leaf assignment: uniform_tree.cpp:198-223.
+    auto assign_leaf = [&](const V3& p) {
+      auto in_range = [](double v, double lo, double hi) { return v >= lo - kTol && v <= hi + kTol; };
+      if (!in_range(p.x, rmin.x, rmax.x) || !in_range(p.y, rmin.y, rmax.y) ||
+          !in_range(p.z, rmin.z, rmax.z))
+        throw std::invalid_argument("Point lies outside the requested root box");
+      if (L == 0) return std::array<int, 3>{0, 0, 0};
+      const int n = 1 << L;
+      const double cw = (2.0 * root_hw) / static_cast<double>(n);
+      auto idx = [&](double v, double lo) {
+        int i = static_cast<int>(std::floor((v - lo) / cw));
+        if (i < 0) i = 0;
+        if (i >= n) i = n - 1;
+        return i;
+      };
+      return std::array<int, 3>{idx(p.x, rmin.x), idx(p.y, rmin.y), idx(p.z, rmin.z)};
+    };
+    // Nodes: uniform_tree.cpp:225-288.
+    nodes.assign(static_cast<std::size_t>(level_offset(L + 1)), Node{});
+    for (int level = 0; level <= L; ++level) {
+      const int n = 1 << level;
+      for (int iz = 0; iz < n; ++iz)
+        for (int iy = 0; iy < n; ++iy)
+          for (int ix = 0; ix < n; ++ix) {
+            Node& nd = nodes[static_cast<std::size_t>(node_index(level, ix, iy, iz))];
+            nd.level = level;
+            nd.ix = ix;
+            nd.iy = iy;
+            nd.iz = iz;
+            nd.parent = level == 0 ? -1 : node_index(level - 1, ix / 2, iy / 2, iz / 2);
+            nd.half_width = root_hw / static_cast<double>(1 << level);
+            const V3 off{((static_cast<double>(ix) + 0.5) / static_cast<double>(n) - 0.5) * 2.0 * root_hw,
+                         ((static_cast<double>(iy) + 0.5) / static_cast<double>(n) - 0.5) * 2.0 * root_hw,
+                         ((static_cast<double>(iz) + 0.5) / static_cast<double>(n) - 0.5) * 2.0 * root_hw};
+            nd.centre = root_centre + off;
+            if (level < L)
+              for (int c = 0; c < 8; ++c)
+                nd.children[c] = node_index(level + 1, 2 * ix + (c & 1), 2 * iy + ((c >> 1) & 1),
+                                            2 * iz + ((c >> 2) & 1));
+          }
+    }
+    // Stable Morton sort: uniform_tree.cpp:290-357.
+    auto sort_points = [&](const std::vector<V3>& orig, std::vector<V3>& sorted, std::vector<int>& perm,
+                           std::vector<int>& inv, std::vector<int>& leaf) {
+      std::vector<std::tuple<std::uint64_t, int, int>> keyed;
+      keyed.reserve(orig.size());
+      for (std::size_t i = 0; i < orig.size(); ++i) {
+        const auto ijk = assign_leaf(orig[i]);
+        keyed.emplace_back(morton_encode(ijk[0], ijk[1], ijk[2]), static_cast<int>(i),
+                           node_index(L, ijk[0], ijk[1], ijk[2]));
+      }
+      std::stable_sort(keyed.begin(), keyed.end(),
+                       [](const auto& a, const auto& b) { return std::get<0>(a) < std::get<0>(b); });
+      sorted.resize(orig.size());
+      perm.resize(orig.size());
+      inv.resize(orig.size());
+      leaf.resize(orig.size());
+      for (std::size_t s = 0; s < keyed.size(); ++s) {
+        const int o = std::get<1>(keyed[s]);
+        sorted[s] = orig[static_cast<std::size_t>(o)];
+        perm[s] = o;
+        inv[static_cast<std::size_t>(o)] = static_cast<int>(s);
+        leaf[s] = std::get<2>(keyed[s]);
+      }
+    };
+    sort_points(sources, src, src_perm, src_inv, src_leaf);
+    sort_points(targets, tgt, tgt_perm, tgt_inv, tgt_leaf);
+    // Ranges: uniform_tree.cpp:359-438 ([n,n) is the empty sentinel).
+    for (Node& nd : nodes) {
+      nd.sb = nd.se = src.size();
+      nd.tb = nd.te = tgt.size();
+    }
+    auto assign_ranges = [&](const std::vector<int>& leaf_of, bool is_src) {
+      std::size_t cur = 0;
+      while (cur < leaf_of.size()) {
+        const int leaf = leaf_of[cur];
+        std::size_t nx = cur + 1;
+        while (nx < leaf_of.size() && leaf_of[nx] == leaf) ++nx;
+        Node& nd = nodes[static_cast<std::size_t>(leaf)];
+        (is_src ? nd.sb : nd.tb) = cur;
+        (is_src ? nd.se : nd.te) = nx;
+        cur = nx;
+      }
+      const std::size_t total = is_src ? src.size() : tgt.size();
+      for (int level = L - 1; level >= 0; --level)
+        for (int idx = level_offset(level); idx < level_offset(level + 1); ++idx) {
+          Node& nd = nodes[static_cast<std::size_t>(idx)];
+          std::size_t lo = total, hi = 0;
+          bool occupied = false;
+          for (const int c : nd.children) {
+            const Node& ch = nodes[static_cast<std::size_t>(c)];
+            const std::size_t b = is_src ? ch.sb : ch.tb, e = is_src ? ch.se : ch.te;
+            if (b == e) continue;
+            occupied = true;
+            lo = std::min(lo, b);
+            hi = std::max(hi, e);
+          }
+          if (!occupied) hi = lo;
+          (is_src ? nd.sb : nd.tb) = lo;
+          (is_src ? nd.se : nd.te) = hi;
+        }
+    };
+    assign_ranges(src_leaf, true);
+    assign_ranges(tgt_leaf, false);
+    // Occupied leaves: uniform_tree.cpp:440-453.
+    for (int idx = level_offset(L); idx < static_cast<int>(nodes.size()); ++idx) {
+      if (nodes[static_cast<std::size_t>(idx)].scount() > 0) occ_src.push_back(idx);
+      if (nodes[static_cast<std::size_t>(idx)].tcount() > 0) occ_tgt.push_back(idx);
+    }
+    // list1 / list2: uniform_tree.cpp:455-515.
+    for (int level = 0; level <= L; ++level) {
+      const int n = 1 << level;
+      for (int idx = level_offset(level); idx < level_offset(level + 1); ++idx) {
+        Node& nd = nodes[static_cast<std::size_t>(idx)];
+        nd.list1.clear();
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+              const int x = nd.ix + dx, y = nd.iy + dy, z = nd.iz + dz;
+              if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n)
+                nd.list1.push_back(node_index(level, x, y, z));
+            }
+        std::sort(nd.list1.begin(), nd.list1.end());
+        if (level == 0) continue;
+        std::vector<int> cand;
+        for (const int pn : nodes[static_cast<std::size_t>(nd.parent)].list1)
+          for (const int c : nodes[static_cast<std::size_t>(pn)].children) cand.push_back(c);
+        std::sort(cand.begin(), cand.end());
+        std::set_difference(cand.begin(), cand.end(), nd.list1.begin(), nd.list1.end(),
+                            std::back_inserter(nd.list2));
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Static plan (src/uniform_fmm.cpp:370-824) and CpuStatic evaluation
+// (src/uniform_fmm.cpp:1065-1457, src/far_field.cpp:248-581,
+// src/static_operators.cpp).  S is the stored/executed scalar.
+// ---------------------------------------------------------------------------
+template <class S>
+struct Entry {
+  int out, in;
+  S val;
+};
+
+template <class S>
+struct Fmm {
+  static constexpr bool kFloat = std::is_same_v<S, float>;
+  Tree tree;
+  Basis basis;
+  double scale{1.0};  // float_coordinate_scale_, uniform_fmm.cpp:379-387
+  std::vector<std::vector<Entry<S>>> p2m;                 // per occupied source leaf
+  std::vector<std::array<std::vector<Entry<S>>, 8>> m2m;  // per level, child class
+  std::vector<std::array<std::vector<Entry<S>>, 8>> l2l;
+  // M2L plan: static_operators.hpp StaticM2LPlan.
+  std::vector<S> matrices, ms, ls;
+  std::vector<int> rows, row_src, row_mid, row_level;
+  std::vector<std::array<int, 3>> class_keys;
+  // L2P field rows per sorted target: 3*n values (static_operators.cpp:300-335).
+  std::vector<S> l2p_field;
+  std::vector<S> l2p_potential;
+  // P2P compact plan (static_operators.cpp:472-498): rows, sources, 6 tensors.
+  std::vector<int> p2p_rows, p2p_src;
+  std::array<std::vector<S>, 6> p2p_t;
+  std::vector<V3> op_src, op_tgt;  // operator coordinates (normalised for FP32)
+  std::vector<int> fixed_ids;
+  bool has_fixed{false};
+  std::vector<int> fixed_self;
+  // State.
+  std::vector<S> M, Lc;
+
+  Fmm(const std::vector<V3>& s, const std::vector<V3>& t, int order, int max_level, bool has_root,
+      V3 c, double hw, const int* ids, int n_ids)
+      : basis(order < 0 ? 0 : order) {
+    if (order < 0) throw std::invalid_argument("UniformFmmOptions.expansion_order must be >= 0");
+    tree.build(s, t, max_level, has_root, c, hw);
+    if (ids) {  // uniform_fmm.cpp:265-295
+      if (static_cast<std::size_t>(n_ids) != t.size())
+        throw std::invalid_argument("fixed_target_source_indices must contain one entry per target");
+      for (int i = 0; i < n_ids; ++i)
+        if (ids[i] < -1 || ids[i] >= static_cast<int>(s.size()))
+          throw std::invalid_argument("fixed_target_source_indices contains an invalid source index");
+      fixed_ids.assign(ids, ids + n_ids);
+      has_fixed = true;
+      fixed_self = sorted_self(fixed_ids);
+    }
+    build_plan();
+  }
+
+  V3 normalise(const V3& p) const {  // uniform_fmm.cpp:393-397
+    return kFloat ? (p - root_centre()) * (1.0 / scale) : p;
+  }
+  V3 root_centre() const { return tree.nodes.empty() ? V3{} : tree.nodes.front().centre; }
+
+  void build_plan() {
+    const auto& nodes = tree.nodes;
+    const int n = basis.size();
+    if (kFloat && !nodes.empty()) scale = 2.0 * nodes.front().half_width;
+    op_src.resize(tree.src.size());
+    op_tgt.resize(tree.tgt.size());
+    for (std::size_t i = 0; i < tree.src.size(); ++i) op_src[i] = normalise(tree.src[i]);
+    for (std::size_t i = 0; i < tree.tgt.size(); ++i) op_tgt[i] = normalise(tree.tgt[i]);
+    // P2M: uniform_fmm.cpp:432-455; static_operators.cpp:186-220.
+    p2m.resize(tree.occ_src.size());
+#pragma omp parallel for schedule(dynamic, 16)
+    for (std::ptrdiff_t o = 0; o < static_cast<std::ptrdiff_t>(tree.occ_src.size()); ++o) {
+      const Node& leaf = nodes[static_cast<std::size_t>(tree.occ_src[static_cast<std::size_t>(o)])];
+      const V3 centre = normalise(leaf.centre);
+      auto& ent = p2m[static_cast<std::size_t>(o)];
+      for (std::size_t j = 0; j < leaf.scount(); ++j) {
+        const V3 dx = op_src[leaf.sb + j] - centre;
+        for (int a = 0; a < n; ++a) {
+          const MI al = basis.a[a];
+          const double sign = degree(al) % 2 == 0 ? 1.0 : -1.0;
+          for (int comp = 0; comp < 3; ++comp) {
+            if (al[comp] == 0) continue;
+            MI sh = al;
+            sh[comp] -= 1;
+            ent.push_back({a, static_cast<int>(3 * j) + comp,
+                           static_cast<S>(sign * monomial_over_factorial(dx, sh))});
+          }
+        }
+      }
+    }
+    // M2M / L2L: uniform_fmm.cpp:457-500; static_operators.cpp:257-298.
+    m2m.resize(static_cast<std::size_t>(tree.L + 1));
+    l2l.resize(static_cast<std::size_t>(tree.L + 1));
+    for (int level = 1; level <= tree.L; ++level) {
+      const double phys = nodes[static_cast<std::size_t>(level_offset(level))].half_width;
+      const double h = kFloat ? phys / scale : phys;
+      for (int cls = 0; cls < 8; ++cls) {
+        const V3 off{(cls & 1) != 0 ? h : -h, (cls & 2) != 0 ? h : -h, (cls & 4) != 0 ? h : -h};
+        const V3 dm = off * -1.0;
+        auto& mm = m2m[static_cast<std::size_t>(level)][cls];
+        for (int a = 0; a < n; ++a)
+          for (int g = 0; g < n; ++g)
+            if (leq(basis.a[g], basis.a[a]))
+              mm.push_back({a,
+                            basis.index({basis.a[a][0] - basis.a[g][0], basis.a[a][1] - basis.a[g][1],
+                                         basis.a[a][2] - basis.a[g][2]}),
+                            static_cast<S>(monomial_over_factorial(dm, basis.a[g]))});
+        auto& ll = l2l[static_cast<std::size_t>(level)][cls];
+        for (int bb = 0; bb < n; ++bb)
+          for (int g = 0; g < n; ++g) {
+            const MI sum{basis.a[bb][0] + basis.a[g][0], basis.a[bb][1] + basis.a[g][1],
+                         basis.a[bb][2] + basis.a[g][2]};
+            if (degree(sum) <= basis.p)
+              ll.push_back({bb, basis.index(sum), static_cast<S>(monomial_over_factorial(off, basis.a[g]))});
+          }
+      }
+    }
+    // M2L class discovery, scaling, matrices, CSR rows: uniform_fmm.cpp:502-604.
+    std::map<std::tuple<int, int, int>, std::vector<std::pair<int, int>>> classes;
+    for (std::size_t ti = 0; ti < nodes.size(); ++ti) {
+      const Node& tn = nodes[ti];
+      if (tn.level == 0 || tn.tcount() == 0) continue;
+      for (const int si : tn.list2) {
+        const Node& sn = nodes[static_cast<std::size_t>(si)];
+        if (sn.scount() == 0) continue;
+        classes[{tn.ix - sn.ix, tn.iy - sn.iy, tn.iz - sn.iz}].emplace_back(si, static_cast<int>(ti));
+      }
+    }
+    const int levels = tree.L + 1;
+    ms.resize(static_cast<std::size_t>(levels) * n);
+    ls.resize(static_cast<std::size_t>(levels) * n);
+    std::vector<double> iwp(static_cast<std::size_t>(basis.p + 2), 1.0);
+    for (int level = 0; level <= tree.L; ++level) {
+      const double phys = 2.0 * nodes[static_cast<std::size_t>(level_offset(level))].half_width;
+      const double w = kFloat ? phys / scale : phys;
+      iwp[0] = 1.0;
+      for (int d = 1; d <= basis.p + 1; ++d) iwp[d] = iwp[d - 1] / w;
+      for (int i = 0; i < n; ++i) {
+        ms[static_cast<std::size_t>(level) * n + i] = static_cast<S>(iwp[degree(basis.a[i])]);
+        ls[static_cast<std::size_t>(level) * n + i] = static_cast<S>(iwp[degree(basis.a[i]) + 1]);
+      }
+    }
+    std::vector<std::tuple<int, int, int>> keys;
+    for (const auto& kv : classes) keys.push_back(kv.first);
+    matrices.resize(keys.size() * static_cast<std::size_t>(n) * n);
+    if (!keys.empty()) {
+      const Jets jets(2 * basis.p);
+#pragma omp parallel for schedule(dynamic, 1)
+      for (std::ptrdiff_t k = 0; k < static_cast<std::ptrdiff_t>(keys.size()); ++k) {
+        const auto [dx, dy, dz] = keys[static_cast<std::size_t>(k)];
+        const auto m = m2l_matrix(basis, jets, V3{double(dx), double(dy), double(dz)});
+        for (std::size_t i = 0; i < m.size(); ++i)
+          matrices[static_cast<std::size_t>(k) * n * n + i] = static_cast<S>(m[i]);
+      }
+    }
+    rows.assign(nodes.size() + 1, 0);
+    std::size_t count = 0;
+    for (const auto& kv : classes) {
+      class_keys.push_back({std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)});
+      for (const auto& [src, tgt] : kv.second) {
+        ++rows[static_cast<std::size_t>(tgt) + 1];
+        ++count;
+      }
+    }
+    std::partial_sum(rows.begin(), rows.end(), rows.begin());
+    row_src.resize(count);
+    row_mid.resize(count);
+    row_level.resize(count);
+    std::vector<int> cursor = rows;
+    int mid = 0;
+    for (const auto& kv : classes) {
+      for (const auto& [src, tgt] : kv.second) {
+        const int slot = cursor[static_cast<std::size_t>(tgt)]++;
+        row_src[static_cast<std::size_t>(slot)] = src;
+        row_mid[static_cast<std::size_t>(slot)] = mid;
+        row_level[static_cast<std::size_t>(slot)] = nodes[static_cast<std::size_t>(tgt)].level;
+      }
+      ++mid;
+    }
+    // L2P rows: uniform_fmm.cpp:663-676; static_operators.cpp:300-335.
+    l2p_field.assign(tree.tgt.size() * 3 * static_cast<std::size_t>(n), S(0));
+    l2p_potential.assign(tree.tgt.size() * static_cast<std::size_t>(n), S(0));
+#pragma omp parallel for schedule(dynamic, 16)
+    for (std::ptrdiff_t o = 0; o < static_cast<std::ptrdiff_t>(tree.occ_tgt.size()); ++o) {
+      const Node& leaf = nodes[static_cast<std::size_t>(tree.occ_tgt[static_cast<std::size_t>(o)])];
+      const V3 centre = normalise(leaf.centre);
+      for (std::size_t t = leaf.tb; t < leaf.te; ++t) {
+        const V3 dx = op_tgt[t] - centre;
+        S* f = &l2p_field[t * 3 * n];
+        S* pot = &l2p_potential[t * n];
+        for (int bb = 0; bb < n; ++bb) {
+          const MI be = basis.a[bb];
+          pot[bb] = static_cast<S>(monomial_over_factorial(dx, be));
+          for (int comp = 0; comp < 3; ++comp)
+            if (be[comp] > 0) {
+              MI sh = be;
+              sh[comp] -= 1;
+              f[comp * n + bb] = static_cast<S>(-monomial_over_factorial(dx, sh));
+            }
+        }
+      }
+    }
+    // P2P pairs (uniform_fmm.cpp:678-710) in (target, source) order, tensor
+    // values from cuboid.cpp:293-307 and static_operators.cpp:421-470.
+    p2p_rows.assign(tree.tgt.size() + 1, 0);
+    for (const int li : tree.occ_tgt) {
+      const Node& leaf = nodes[static_cast<std::size_t>(li)];
+      std::size_t cnt = 0;
+      for (const int nb : leaf.list1) cnt += nodes[static_cast<std::size_t>(nb)].scount();
+      for (std::size_t t = leaf.tb; t < leaf.te; ++t) p2p_rows[t + 1] = static_cast<int>(cnt);
+    }
+    std::partial_sum(p2p_rows.begin(), p2p_rows.end(), p2p_rows.begin());
+    const std::size_t pairs = static_cast<std::size_t>(p2p_rows.back());
+    p2p_src.resize(pairs);
+    for (auto& v : p2p_t) v.resize(pairs);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (std::ptrdiff_t o = 0; o < static_cast<std::ptrdiff_t>(tree.occ_tgt.size()); ++o) {
+      const Node& leaf = nodes[static_cast<std::size_t>(tree.occ_tgt[static_cast<std::size_t>(o)])];
+      for (std::size_t t = leaf.tb; t < leaf.te; ++t) {
+        std::size_t e = static_cast<std::size_t>(p2p_rows[t]);
+        for (const int nb : leaf.list1) {
+          const Node& sn = nodes[static_cast<std::size_t>(nb)];
+          for (std::size_t s = sn.sb; s < sn.se; ++s, ++e) {
+            p2p_src[e] = static_cast<int>(s);
+            double v[6];
+            pair_tensor(op_tgt[t], op_src[s], v);
+            for (int c = 0; c < 6; ++c) p2p_t[static_cast<std::size_t>(c)][e] = static_cast<S>(v[c]);
+          }
+        }
+      }
+    }
+    M.assign(nodes.size() * static_cast<std::size_t>(n), S(0));
+    Lc.assign(nodes.size() * static_cast<std::size_t>(n), S(0));
+  }
+
+  // cuboid.cpp:293-307 (point dipole) and the coincident-pair NaN rule of
+  // static_operators.cpp:435-446.
+  static void pair_tensor(const V3& t, const V3& s, double* v) {
+    const V3 r = t - s;
+    if (dot(r, r) == 0.0) {
+      for (int c = 0; c < 6; ++c) v[c] = std::numeric_limits<double>::quiet_NaN();
+      return;
+    }
+    const double r2 = dot(r, r);
+    const double inverse_r = 1.0 / std::sqrt(r2);
+    const double inverse_r3 = inverse_r / r2;
+    const double diagonal = inverse_r3 / (4.0 * std::numbers::pi);
+    const double common = 3.0 * diagonal / r2;
+    v[0] = common * r.x * r.x - diagonal;
+    v[1] = common * r.x * r.y;
+    v[2] = common * r.x * r.z;
+    v[3] = common * r.y * r.y - diagonal;
+    v[4] = common * r.y * r.z;
+    v[5] = common * r.z * r.z - diagonal;
+  }
+  // static_operators.cpp:455-463 potential row of one pair.
+  static void pair_potential(const V3& t, const V3& s, double* v) {
+    const V3 r = t - s;
+    if (dot(r, r) == 0.0) {
+      v[0] = v[1] = v[2] = std::numeric_limits<double>::quiet_NaN();
+      return;
+    }
+    const double r2 = dot(r, r);
+    const double ps = r2 == 0.0 ? 0.0 : 1.0 / (4.0 * std::numbers::pi * r2 * std::sqrt(r2));
+    v[0] = ps * r.x;
+    v[1] = ps * r.y;
+    v[2] = ps * r.z;
+  }
+
+  // uniform_fmm.cpp:1028-1043
+  std::vector<int> sorted_self(const std::vector<int>& ids) const {
+    std::vector<int> out(tree.tgt.size(), -1);
+    for (std::size_t t = 0; t < ids.size(); ++t) {
+      const int o = ids[static_cast<std::size_t>(tree.tgt_perm[t])];
+      if (o >= 0) out[t] = tree.src_inv[static_cast<std::size_t>(o)];
+    }
+    return out;
+  }
+
+  static void apply_op(const std::vector<Entry<S>>& ops, const S* in, S* out) {
+    for (const auto& e : ops) out[e.out] += e.val * in[e.in];  // static_operators.cpp:963-980/1163-1176
+  }
+
+  void evaluate(const std::vector<V3>& moments, const std::vector<int>& ids_in, int output, double* H_out,
+                double* phi_out) {
+    const auto& nodes = tree.nodes;
+    const int n = basis.size();
+    const std::size_t nt = tree.tgt.size();
+    // resolve_self_indices + validation: uniform_fmm.cpp:1045-1100.
+    std::vector<int> ids = ids_in;
+    if (has_fixed) {
+      if (ids.empty())
+        ids = fixed_ids;
+      else if (ids != fixed_ids)
+        throw std::invalid_argument("fixed target-to-source identity map changed; rebuild the FMM plan");
+    }
+    if (!ids.empty() && ids.size() != nt)
+      throw std::invalid_argument("UniformFmm::evaluate identity map has incorrect length");
+    for (const int v : ids)
+      if (v < -1 || v >= static_cast<int>(moments.size()))
+        throw std::invalid_argument("UniformFmm::evaluate identity map contains an invalid index");
+    if (moments.size() != tree.src.size())
+      throw std::invalid_argument("UniformFmm::upward_pass requires one dipole moment per source position");
+    const std::vector<int> self = sorted_self(ids);
+    const bool want_field = (output & 2) != 0, want_pot = (output & 1) != 0;
+
+    // prepare_moments[_float]: far_field.cpp:248-275 / :334-365.
+    std::fill(M.begin(), M.end(), S(0));
+    std::vector<std::array<S, 3>> sm(tree.src.size());
+#pragma omp parallel for schedule(static)
+    for (std::ptrdiff_t s = 0; s < static_cast<std::ptrdiff_t>(sm.size()); ++s) {
+      const V3 m = moments[static_cast<std::size_t>(tree.src_perm[static_cast<std::size_t>(s)])];
+      if constexpr (kFloat) {
+        sm[static_cast<std::size_t>(s)] = {static_cast<float>(m.x / scale / scale / scale),
+                                           static_cast<float>(m.y / scale / scale / scale),
+                                           static_cast<float>(m.z / scale / scale / scale)};
+      } else {
+        sm[static_cast<std::size_t>(s)] = {m.x, m.y, m.z};
+      }
+    }
+    // P2M: far_field.cpp:277-301 / :367-393.
+#pragma omp parallel for schedule(static)
+    for (std::ptrdiff_t o = 0; o < static_cast<std::ptrdiff_t>(tree.occ_src.size()); ++o) {
+      const int li = tree.occ_src[static_cast<std::size_t>(o)];
+      const Node& leaf = nodes[static_cast<std::size_t>(li)];
+      S* Mo = &M[static_cast<std::size_t>(li) * n];
+      for (const auto& e : p2m[static_cast<std::size_t>(o)]) {
+        const auto& mom = sm[leaf.sb + static_cast<std::size_t>(e.in / 3)];
+        const S comp = e.in % 3 == 0 ? mom[0] : (e.in % 3 == 1 ? mom[1] : mom[2]);
+        Mo[e.out] += e.val * comp;
+      }
+    }
+    // M2M: far_field.cpp:303-331 / :395-446.
+    for (int level = tree.L - 1; level >= 0; --level) {
+#pragma omp parallel for schedule(static)
+      for (int pi = level_offset(level); pi < level_offset(level + 1); ++pi) {
+        const Node& par = nodes[static_cast<std::size_t>(pi)];
+        if (par.scount() == 0) continue;
+        for (const int ci : par.children) {
+          const Node& ch = nodes[static_cast<std::size_t>(ci)];
+          if (ch.scount() == 0) continue;
+          const int cls = (ch.ix & 1) | ((ch.iy & 1) << 1) | ((ch.iz & 1) << 2);
+          apply_op(m2m[static_cast<std::size_t>(ch.level)][cls], &M[static_cast<std::size_t>(ci) * n],
+                   &M[static_cast<std::size_t>(pi) * n]);
+        }
+      }
+    }
+    // Downward pass: far_field.cpp:453-581 (L2L then M2L per level).
+    std::fill(Lc.begin(), Lc.end(), S(0));
+    for (int level = 1; level <= tree.L; ++level) {
+      const int b = level_offset(level), e = level_offset(level + 1);
+#pragma omp parallel for schedule(static)
+      for (int ti = b; ti < e; ++ti) {
+        const Node& tn = nodes[static_cast<std::size_t>(ti)];
+        if (tn.tcount() == 0) continue;
+        const int cls = (tn.ix & 1) | ((tn.iy & 1) << 1) | ((tn.iz & 1) << 2);
+        apply_op(l2l[static_cast<std::size_t>(tn.level)][cls], &Lc[static_cast<std::size_t>(tn.parent) * n],
+                 &Lc[static_cast<std::size_t>(ti) * n]);
+      }
+      // apply_static_m2l_plan: static_operators.cpp:16-102.
+      const S* msl = &ms[static_cast<std::size_t>(level) * n];
+      const S* lsl = &ls[static_cast<std::size_t>(level) * n];
+      const std::ptrdiff_t outputs = static_cast<std::ptrdiff_t>(e - b) * n;
+#pragma omp parallel for schedule(static)
+      for (std::ptrdiff_t o = 0; o < outputs; ++o) {
+        const int target = b + static_cast<int>(o / n);
+        const int beta = static_cast<int>(o % n);
+        S value = S(0);
+        for (int it = rows[static_cast<std::size_t>(target)]; it < rows[static_cast<std::size_t>(target) + 1];
+             ++it) {
+          const S* col = matrices.data() + static_cast<std::size_t>(row_mid[static_cast<std::size_t>(it)]) * n * n + beta;
+          const S* sM = M.data() + static_cast<std::size_t>(row_src[static_cast<std::size_t>(it)]) * n;
+          for (int alpha = 0; alpha < n; ++alpha) value += col[static_cast<std::size_t>(alpha) * n] * msl[alpha] * sM[alpha];
+        }
+        Lc[static_cast<std::size_t>(target) * n + beta] += lsl[beta] * value;
+      }
+    }
+    // L2P: uniform_fmm.cpp:1160-1186 / :1360-1387; static_operators.cpp:1178-1198.
+    std::vector<std::array<S, 4>> res(nt, std::array<S, 4>{S(0), S(0), S(0), S(0)});
+#pragma omp parallel for schedule(static)
+    for (std::ptrdiff_t o = 0; o < static_cast<std::ptrdiff_t>(tree.occ_tgt.size()); ++o) {
+      const int li = tree.occ_tgt[static_cast<std::size_t>(o)];
+      const Node& leaf = nodes[static_cast<std::size_t>(li)];
+      const S* Lo = &Lc[static_cast<std::size_t>(li) * n];
+      for (std::size_t t = leaf.tb; t < leaf.te; ++t) {
+        const S* f = &l2p_field[t * 3 * n];
+        const S* pot = &l2p_potential[t * n];
+        S phi = S(0), hx = S(0), hy = S(0), hz = S(0);
+        for (int i = 0; i < n; ++i) {
+          if (want_pot) phi += pot[i] * Lo[i];
+          if (want_field) {
+            hx += f[i] * Lo[i];
+            hy += f[n + i] * Lo[i];
+            hz += f[2 * n + i] * Lo[i];
+          }
+        }
+        res[t] = {phi, hx, hy, hz};
+      }
+    }
+    // Near field.
+    if (want_field) {
+      if constexpr (kFloat) {
+        // apply_static_p2p_compact_plan FP32: static_operators.cpp:1200-1237.
+#pragma omp parallel for schedule(static)
+        for (std::ptrdiff_t t = 0; t < static_cast<std::ptrdiff_t>(nt); ++t) {
+          const int sf = self[static_cast<std::size_t>(t)];
+          F3 field{};
+          for (int e = p2p_rows[static_cast<std::size_t>(t)]; e < p2p_rows[static_cast<std::size_t>(t) + 1]; ++e) {
+            const std::size_t i = static_cast<std::size_t>(e);
+            const int s = p2p_src[i];
+            if (s == sf) continue;
+            const F3 m{sm[static_cast<std::size_t>(s)][0], sm[static_cast<std::size_t>(s)][1],
+                       sm[static_cast<std::size_t>(s)][2]};
+            field.x += p2p_t[0][i] * m.x + p2p_t[1][i] * m.y + p2p_t[2][i] * m.z;
+            field.y += p2p_t[1][i] * m.x + p2p_t[3][i] * m.y + p2p_t[4][i] * m.z;
+            field.z += p2p_t[2][i] * m.x + p2p_t[4][i] * m.y + p2p_t[5][i] * m.z;
+          }
+          float nx = 0.0F, ny = 0.0F, nz = 0.0F;
+          nx += field.x;
+          ny += field.y;
+          nz += field.z;
+          auto& r = res[static_cast<std::size_t>(t)];
+          r[1] += nx;
+          r[2] += ny;
+          r[3] += nz;
+        }
+      } else {
+        // apply_static_p2p_compact_plan FP64: static_operators.cpp:686-725.
+#pragma omp parallel for schedule(static)
+        for (std::ptrdiff_t t = 0; t < static_cast<std::ptrdiff_t>(nt); ++t) {
+          double hx = 0.0, hy = 0.0, hz = 0.0;
+          const int sf = self[static_cast<std::size_t>(t)];
+          const int b = p2p_rows[static_cast<std::size_t>(t)], e = p2p_rows[static_cast<std::size_t>(t) + 1];
+#pragma omp simd reduction(+ : hx, hy, hz)
+          for (int k = b; k < e; ++k) {
+            const int s = p2p_src[static_cast<std::size_t>(k)];
+            if (s == sf) continue;
+            const auto& m = sm[static_cast<std::size_t>(s)];
+            const std::size_t i = static_cast<std::size_t>(k);
+            hx += p2p_t[0][i] * m[0] + p2p_t[1][i] * m[1] + p2p_t[2][i] * m[2];
+            hy += p2p_t[1][i] * m[0] + p2p_t[3][i] * m[1] + p2p_t[4][i] * m[2];
+            hz += p2p_t[2][i] * m[0] + p2p_t[4][i] * m[1] + p2p_t[5][i] * m[2];
+          }
+          double nx = 0.0, ny = 0.0, nz = 0.0;
+          nx += hx;
+          ny += hy;
+          nz += hz;
+          auto& r = res[static_cast<std::size_t>(t)];
+          r[1] += nx;
+          r[2] += ny;
+          r[3] += nz;
+        }
+      }
+    }
+    if (want_pot) {
+      if constexpr (kFloat) {
+        // FP32 potential from the compact plan rows: uniform_fmm.cpp:1414-1437.
+#pragma omp parallel for schedule(static)
+        for (std::ptrdiff_t t = 0; t < static_cast<std::ptrdiff_t>(nt); ++t) {
+          float potential = 0.0F;
+          const int sf = self[static_cast<std::size_t>(t)];
+          for (int e = p2p_rows[static_cast<std::size_t>(t)]; e < p2p_rows[static_cast<std::size_t>(t) + 1]; ++e) {
+            const int s = p2p_src[static_cast<std::size_t>(e)];
+            if (s == sf) continue;
+            double pv[3];
+            pair_potential(op_tgt[static_cast<std::size_t>(t)], op_src[static_cast<std::size_t>(s)], pv);
+            const auto& m = sm[static_cast<std::size_t>(s)];
+            potential += static_cast<float>(pv[0]) * m[0] + static_cast<float>(pv[1]) * m[1] +
+                         static_cast<float>(pv[2]) * m[2];
+          }
+          res[static_cast<std::size_t>(t)][0] += potential;
+        }
+      } else {
+        // evaluate_reference_near_field potential: near_field.cpp:30-80 with
+        // p2p_dipole_pair (operators.cpp:178-199).
+#pragma omp parallel for schedule(static)
+        for (std::ptrdiff_t o = 0; o < static_cast<std::ptrdiff_t>(tree.occ_tgt.size()); ++o) {
+          const Node& leaf = nodes[static_cast<std::size_t>(tree.occ_tgt[static_cast<std::size_t>(o)])];
+          for (std::size_t t = leaf.tb; t < leaf.te; ++t) {
+            const int sf = self[t];
+            for (const int nb : leaf.list1) {
+              const Node& sn = nodes[static_cast<std::size_t>(nb)];
+              if (sn.scount() == 0) continue;
+              double phi = 0.0;
+              for (std::size_t s = sn.sb; s < sn.se; ++s) {
+                if (static_cast<int>(s) == sf) continue;
+                const V3 r = tree.tgt[t] - tree.src[s];
+                const double r2 = dot(r, r);
+                const double rinv = 1.0 / std::sqrt(r2);
+                const double rinv3 = rinv * rinv * rinv;
+                const double c = 1.0 / (4.0 * std::numbers::pi);
+                const auto& m = sm[s];
+                const double mdr = m[0] * r.x + m[1] * r.y + m[2] * r.z;
+                phi += c * mdr * rinv3;
+              }
+              res[t][0] += phi;
+            }
+          }
+        }
+      }
+    }
+    // Unsort: uniform_fmm.cpp:1239-1252 / :1439-1452.
+    for (std::size_t s = 0; s < nt; ++s) {
+      const std::size_t o = static_cast<std::size_t>(tree.tgt_perm[s]);
+      S phi = res[s][0];
+      if constexpr (kFloat) phi *= static_cast<float>(scale);
+      H_out[3 * o] = static_cast<double>(want_field ? res[s][1] : S(0));
+      H_out[3 * o + 1] = static_cast<double>(want_field ? res[s][2] : S(0));
+      H_out[3 * o + 2] = static_cast<double>(want_field ? res[s][3] : S(0));
+      if (phi_out) phi_out[o] = static_cast<double>(phi);
+    }
+  }
+};
+
+// validation.cpp:71-111 with operators.cpp:178-220 (FP64 direct sum).
+void direct(const std::vector<V3>& t, const std::vector<V3>& s, const std::vector<V3>& m, const int* ids,
+            int output, double* H, double* phi) {
+  const double c = 1.0 / (4.0 * std::numbers::pi);
+#pragma omp parallel for schedule(static)
+  for (std::ptrdiff_t i = 0; i < static_cast<std::ptrdiff_t>(t.size()); ++i) {
+    const int self = ids ? ids[i] : -1;
+    double ph = 0.0;
+    V3 h{};
+    for (std::size_t j = 0; j < s.size(); ++j) {
+      if (static_cast<int>(j) == self) continue;
+      const V3 r = t[static_cast<std::size_t>(i)] - s[j];
+      const double r2 = dot(r, r);
+      const double rinv = 1.0 / std::sqrt(r2);
+      const double rinv3 = rinv * rinv * rinv;
+      const double mdr = dot(m[j], r);
+      double pphi = 0.0;
+      V3 pH{};
+      if (output & 1) pphi = c * mdr * rinv3;
+      if (output & 2) {
+        const double rinv5 = rinv3 * rinv * rinv;
+        pH = (r * (3.0 * mdr * rinv5) - m[j] * rinv3) * c;
+      }
+      ph += pphi;
+      h = h + pH;
+    }
+    H[3 * i] = h.x;
+    H[3 * i + 1] = h.y;
+    H[3 * i + 2] = h.z;
+    if (phi) phi[i] = ph;
+  }
+}
+
+std::vector<V3> to_v3(const double* p, int n) {
+  std::vector<V3> out(static_cast<std::size_t>(n));
+  for (int i = 0; i < n; ++i) out[static_cast<std::size_t>(i)] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+  return out;
+}
+
+struct Handle {
+  int precision{32};
+  Fmm<float>* f32{nullptr};
+  Fmm<double>* f64{nullptr};
+  ~Handle() {
+    delete f32;
+    delete f64;
+  }
+  template <class Fn>
+  auto visit(Fn&& fn) {
+    return precision == 32 ? fn(*f32) : fn(*f64);
+  }
+};
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return 2;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
+
+}  // namespace
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_last_error.c_str(); }
+
+void* oracle_fmm_create(const double* src, int ns, const double* tgt, int nt, int order, int max_level,
+                        int precision, int has_root, double cx, double cy, double cz, double root_half_width,
+                        const int* fixed_ids, int n_fixed) {
+  Handle* out = nullptr;
+  const int rc = guarded([&] {
+    auto h = std::make_unique<Handle>();
+    h->precision = precision == 32 ? 32 : 64;
+    const auto s = to_v3(src, ns);
+    const auto t = to_v3(tgt, nt);
+    if (h->precision == 32)
+      h->f32 = new Fmm<float>(s, t, order, max_level, has_root != 0, V3{cx, cy, cz}, root_half_width, fixed_ids,
+                              n_fixed);
+    else
+      h->f64 = new Fmm<double>(s, t, order, max_level, has_root != 0, V3{cx, cy, cz}, root_half_width, fixed_ids,
+                               n_fixed);
+    out = h.release();
+  });
+  return rc == 0 ? out : nullptr;
+}
+
+void oracle_fmm_destroy(void* h) { delete static_cast<Handle*>(h); }
+
+int oracle_fmm_evaluate(void* hv, const double* moments, int ns, const int* ids, int n_ids, int output,
+                        double* H_out, double* phi_out) {
+  auto* h = static_cast<Handle*>(hv);
+  return guarded([&] {
+    const auto m = to_v3(moments, ns);
+    std::vector<int> id;
+    if (ids) id.assign(ids, ids + n_ids);
+    h->visit([&](auto& f) {
+      f.evaluate(m, id, output, H_out, phi_out);
+      return 0;
+    });
+  });
+}
+
+int oracle_fmm_coefficients(void* hv, int which, int node, double* out) {
+  auto* h = static_cast<Handle*>(hv);
+  return guarded([&] {
+    h->visit([&](auto& f) {
+      const int n = f.basis.size();
+      const auto& v = which == 0 ? f.M : f.Lc;
+      for (int i = 0; i < n; ++i) out[i] = static_cast<double>(v[static_cast<std::size_t>(node) * n + i]);
+      return 0;
+    });
+  });
+}
+
+void oracle_fmm_sizes(void* hv, long long* out) {
+  auto* h = static_cast<Handle*>(hv);
+  h->visit([&](auto& f) {
+    long long l1 = 0, l2 = 0;
+    for (const auto& nd : f.tree.nodes) {
+      l1 += static_cast<long long>(nd.list1.size());
+      l2 += static_cast<long long>(nd.list2.size());
+    }
+    out[0] = static_cast<long long>(f.tree.nodes.size());
+    out[1] = static_cast<long long>(f.tree.src.size());
+    out[2] = static_cast<long long>(f.tree.tgt.size());
+    out[3] = l1;
+    out[4] = l2;
+    out[5] = f.basis.size();
+    return 0;
+  });
+}
+
+void oracle_fmm_root(void* hv, double* out) {
+  auto* h = static_cast<Handle*>(hv);
+  h->visit([&](auto& f) {
+    out[0] = f.tree.root_centre.x;
+    out[1] = f.tree.root_centre.y;
+    out[2] = f.tree.root_centre.z;
+    out[3] = f.tree.root_hw;
+    return 0;
+  });
+}
+
+long long oracle_fmm_array_size(void* hv, const char* name) {
+  auto* h = static_cast<Handle*>(hv);
+  const std::string n(name);
+  return h->visit([&](auto& f) -> long long {
+    if (n == "m2l_rows") return static_cast<long long>(f.rows.size());
+    if (n == "m2l_sources" || n == "m2l_matrix_ids") return static_cast<long long>(f.row_src.size());
+    if (n == "m2l_class_keys") return static_cast<long long>(3 * f.class_keys.size());
+    if (n == "p2p_rows") return static_cast<long long>(f.p2p_rows.size());
+    return -1;
+  });
+}
+
+int oracle_fmm_tree_ints(void* hv, const char* name, long long* out) {
+  auto* h = static_cast<Handle*>(hv);
+  return guarded([&] {
+    const std::string n(name);
+    h->visit([&](auto& f) {
+      auto copy = [&](const std::vector<int>& v) {
+        for (std::size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+      };
+      const auto& tr = f.tree;
+      if (n == "source_permutation") copy(tr.src_perm);
+      else if (n == "source_inverse_permutation") copy(tr.src_inv);
+      else if (n == "target_permutation") copy(tr.tgt_perm);
+      else if (n == "target_inverse_permutation") copy(tr.tgt_inv);
+      else if (n == "occupied_source_leaves") copy(tr.occ_src);
+      else if (n == "occupied_target_leaves") copy(tr.occ_tgt);
+      else if (n == "m2l_rows") copy(f.rows);
+      else if (n == "m2l_sources") copy(f.row_src);
+      else if (n == "m2l_matrix_ids") copy(f.row_mid);
+      else if (n == "p2p_rows") copy(f.p2p_rows);
+      else if (n == "m2l_class_keys") {
+        for (std::size_t i = 0; i < f.class_keys.size(); ++i)
+          for (int c = 0; c < 3; ++c) out[3 * i + c] = f.class_keys[i][c];
+      } else if (n == "node_ranges") {
+        std::size_t i = 0;
+        for (const auto& nd : tr.nodes) {
+          out[i++] = static_cast<long long>(nd.sb);
+          out[i++] = static_cast<long long>(nd.se);
+          out[i++] = static_cast<long long>(nd.tb);
+          out[i++] = static_cast<long long>(nd.te);
+        }
+      } else if (n == "list1" || n == "list2") {
+        const bool first = n == "list1";
+        std::size_t off = 0;
+        out[0] = 0;
+        for (std::size_t k = 0; k < tr.nodes.size(); ++k) {
+          off += (first ? tr.nodes[k].list1 : tr.nodes[k].list2).size();
+          out[k + 1] = static_cast<long long>(off);
+        }
+        std::size_t i = tr.nodes.size() + 1;
+        for (const auto& nd : tr.nodes)
+          for (const int v : (first ? nd.list1 : nd.list2)) out[i++] = v;
+      } else {
+        throw std::invalid_argument("unknown tree array " + n);
+      }
+      return 0;
+    });
+  });
+}
+
+int oracle_m2l_matrix(int order, double dx, double dy, double dz, double* out) {
+  return guarded([&] {
+    const Basis b(order);
+    const Jets j(2 * order);
+    const auto m = m2l_matrix(b, j, V3{dx, dy, dz});
+    std::memcpy(out, m.data(), m.size() * sizeof(double));
+  });
+}
+
+int oracle_laplace_derivatives(int order, double x, double y, double z, double* out) {
+  return guarded([&] {
+    const Jets j(order);
+    const auto d = j.laplace_raw(V3{x, y, z});
+    std::memcpy(out, d.data(), d.size() * sizeof(double));
+  });
+}
+
+int oracle_direct(const double* tgt, int nt, const double* src, int ns, const double* moments, const int* ids,
+                  int output, double* H_out, double* phi_out) {
+  return guarded([&] { direct(to_v3(tgt, nt), to_v3(src, ns), to_v3(moments, ns), ids, output, H_out, phi_out); });
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+/*
+ * Test infrastructure only -- the oracle.
+ *
+ * A CPU restatement of the reference's CpuStatic uniform-FMM path (tree,
+ * static plan, FP32/FP64 evaluation) and of its FP64 direct sum.  It is the
+ * checker for the HipFull device path and the "port" CPU baseline: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product never links or calls it.
+ *
+ * Every function in cpustatic.cpp cites the reference file:line it restates.
+ * Parity of this restatement is pinned in tests/test_oracle.py against the
+ * reference compiled from its own sources (oracle/_ref), against closed forms
+ * (on-axis dipole fields, Laplace derivatives), and against the FP64 direct
+ * sum (depth-0 FMM equals direct).
+ *
+ * Status codes: 0 ok, 1 runtime error, 2 invalid argument (message in
+ * oracle_last_error()).
+ */
+#ifndef DIPFMM_ORACLE_CPUSTATIC_H
+#define DIPFMM_ORACLE_CPUSTATIC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* oracle_last_error(void);
+
+/* precision: 32 or 64; root_half_width <= 0 infers the root. */
+void* oracle_fmm_create(const double* src, int ns, const double* tgt, int nt, int order,
+                        int max_level, int precision, int has_root, double cx, double cy,
+                        double cz, double root_half_width, const int* fixed_ids, int n_fixed);
+void oracle_fmm_destroy(void* h);
+
+/* output: 1 potential, 2 field, 3 both.  H_out nt*3, phi_out nt (nullable). */
+int oracle_fmm_evaluate(void* h, const double* moments, int ns, const int* ids, int n_ids,
+                        int output, double* H_out, double* phi_out);
+
+/* which: 0 multipole, 1 local (state after the last evaluate). */
+int oracle_fmm_coefficients(void* h, int which, int node, double* out);
+
+/* [node_count, n_sources, n_targets, list1_total, list2_total, basis_size] */
+void oracle_fmm_sizes(void* h, long long* out);
+/* [cx, cy, cz, half_width] */
+void oracle_fmm_root(void* h, double* out);
+/* Named integer arrays: source/target (inverse) permutation, occupied leaves,
+ * node_ranges (4 per node), list1/list2 (CSR offsets then entries),
+ * m2l_rows (CSR offsets), m2l_sources, m2l_matrix_ids, m2l_class_keys (3 per
+ * class, in matrix-id order). */
+int oracle_fmm_tree_ints(void* h, const char* name, long long* out);
+long long oracle_fmm_array_size(void* h, const char* name);
+
+int oracle_m2l_matrix(int order, double dx, double dy, double dz, double* out);
+int oracle_laplace_derivatives(int order, double x, double y, double z, double* out);
+int oracle_direct(const double* tgt, int nt, const double* src, int ns, const double* moments,
+                  const int* ids, int output, double* H_out, double* phi_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,232 @@
+"""Test infrastructure only -- Python access to the CPU oracle.
+
+Two libraries share one C interface (see cpustatic.h):
+
+* ``liboracle.so``: the repository's restatement of the reference CpuStatic
+  path (oracle/cpustatic.cpp);
+* ``_ref/libcdfmm_ref.so``: the reference's own sources compiled by
+  oracle/Makefile, wrapped by oracle/ref_shim.cpp (present only where the
+  reference could be built).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this module; the product never does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libcdfmm_ref.so")
+REFERENCE_ROOT = "/root/reference"
+
+_P = ctypes.POINTER
+_DP = _P(ctypes.c_double)
+_IP = _P(ctypes.c_int)
+_LP = _P(ctypes.c_longlong)
+
+
+def build(reference: bool = True) -> None:
+    """Compile the restatement, and the reference where its sources exist."""
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    if reference and os.path.isdir(os.path.join(REFERENCE_ROOT, "src")):
+        subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REFERENCE_ROOT}"], check=True)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class _Lib:
+    def __init__(self, path: str, prefix: str):
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        L = self.lib
+        f = getattr(L, f"{prefix}_fmm_create")
+        f.restype = ctypes.c_void_p
+        f.argtypes = [_DP, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                      _IP, ctypes.c_int]
+        getattr(L, f"{prefix}_fmm_destroy").argtypes = [ctypes.c_void_p]
+        f = getattr(L, f"{prefix}_fmm_evaluate")
+        f.argtypes = [ctypes.c_void_p, _DP, ctypes.c_int, _IP, ctypes.c_int, ctypes.c_int, _DP, _DP]
+        getattr(L, f"{prefix}_fmm_coefficients").argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _DP]
+        getattr(L, f"{prefix}_fmm_sizes").argtypes = [ctypes.c_void_p, _LP]
+        getattr(L, f"{prefix}_fmm_root").argtypes = [ctypes.c_void_p, _DP]
+        getattr(L, f"{prefix}_fmm_tree_ints").argtypes = [ctypes.c_void_p, ctypes.c_char_p, _LP]
+        getattr(L, f"{prefix}_last_error").restype = ctypes.c_char_p
+        getattr(L, f"{prefix}_m2l_matrix").argtypes = [ctypes.c_int] + [ctypes.c_double] * 3 + [_DP]
+        getattr(L, f"{prefix}_laplace_derivatives").argtypes = [ctypes.c_int] + [ctypes.c_double] * 3 + [_DP]
+        getattr(L, f"{prefix}_direct").argtypes = [_DP, ctypes.c_int, _DP, ctypes.c_int, _DP, _IP,
+                                                   ctypes.c_int, _DP, _DP]
+        if hasattr(L, f"{prefix}_fmm_array_size"):
+            f = getattr(L, f"{prefix}_fmm_array_size")
+            f.restype = ctypes.c_longlong
+            f.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+
+    def fn(self, name):
+        return getattr(self.lib, f"{self.prefix}_{name}")
+
+    def check(self, rc: int) -> None:
+        if rc == 0:
+            return
+        msg = self.fn("last_error")().decode()
+        raise (ValueError if rc == 2 else RuntimeError)(msg)
+
+
+_LIBS: dict[str, _Lib] = {}
+
+
+def library(kind: str = "oracle") -> _Lib:
+    """``kind`` is ``"oracle"`` (restatement) or ``"ref"`` (compiled reference)."""
+    if kind not in _LIBS:
+        if kind == "oracle":
+            _LIBS[kind] = _Lib(ORACLE_SO, "oracle")
+        elif kind == "ref":
+            _LIBS[kind] = _Lib(REF_SO, "ref")
+        else:
+            raise KeyError(kind)
+    return _LIBS[kind]
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+class CpuFmm:
+    """CpuStatic evaluator (restatement or reference) over one fixed geometry."""
+
+    def __init__(self, sources, targets, order: int, max_level: int, precision: int = 32,
+                 root_centre=None, root_half_width: float | None = None, fixed_ids=None,
+                 kind: str = "oracle"):
+        self.lib = library(kind)
+        self._src = np.ascontiguousarray(sources, dtype=np.float64)
+        self._tgt = np.ascontiguousarray(targets, dtype=np.float64)
+        ids = None if fixed_ids is None else np.ascontiguousarray(fixed_ids, dtype=np.int32)
+        c = (0.0, 0.0, 0.0) if root_centre is None else tuple(float(v) for v in root_centre)
+        h = self.lib.fn("fmm_create")(
+            _dp(self._src), len(self._src), _dp(self._tgt), len(self._tgt), order, max_level, precision,
+            0 if root_centre is None else 1, c[0], c[1], c[2],
+            -1.0 if root_half_width is None else float(root_half_width),
+            None if ids is None else ids.ctypes.data_as(_IP), 0 if ids is None else len(ids))
+        if not h:
+            msg = self.lib.fn("last_error")().decode()
+            raise ValueError(msg) if "must" in msg or "invalid" in msg or "outside" in msg else RuntimeError(msg)
+        self.handle = ctypes.c_void_p(h)
+        self.precision = precision
+        self.order = order
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.fn("fmm_destroy")(h)
+            self.handle = None
+
+    def evaluate(self, moments, ids=None, output: str = "field"):
+        flags = {"potential": 1, "field": 2, "both": 3}[output]
+        m = np.ascontiguousarray(moments, dtype=np.float64)
+        nt = len(self._tgt)
+        H = np.zeros((nt, 3))
+        phi = np.zeros(nt)
+        idv = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        self.lib.check(self.lib.fn("fmm_evaluate")(
+            self.handle, _dp(m), len(m), None if idv is None else idv.ctypes.data_as(_IP),
+            0 if idv is None else len(idv), flags, _dp(H), _dp(phi)))
+        return {"H": H, "phi": phi}
+
+    def sizes(self):
+        out = np.zeros(6, dtype=np.int64)
+        self.lib.fn("fmm_sizes")(self.handle, out.ctypes.data_as(_LP))
+        return dict(zip(["nodes", "sources", "targets", "list1", "list2", "basis"], out.tolist()))
+
+    def root(self):
+        out = np.zeros(4)
+        self.lib.fn("fmm_root")(self.handle, _dp(out))
+        return out
+
+    def tree_ints(self, name: str, size: int) -> np.ndarray:
+        out = np.zeros(size, dtype=np.int64)
+        self.lib.check(self.lib.fn("fmm_tree_ints")(self.handle, name.encode(), out.ctypes.data_as(_LP)))
+        return out
+
+    def array(self, name: str) -> np.ndarray:
+        """Plan arrays by name, sized from the plan itself."""
+        s = self.sizes()
+        nn, ns, nt = s["nodes"], s["sources"], s["targets"]
+        fixed = {
+            "source_permutation": ns, "source_inverse_permutation": ns,
+            "target_permutation": nt, "target_inverse_permutation": nt,
+            "node_ranges": 4 * nn, "list1": nn + 1 + s["list1"], "list2": nn + 1 + s["list2"],
+        }
+        if name in fixed:
+            return self.tree_ints(name, fixed[name])
+        if name in ("occupied_source_leaves", "occupied_target_leaves"):
+            r = self.tree_ints("node_ranges", 4 * nn).reshape(-1, 4)
+            leaves = r[(8 ** self.max_level() - 1) // 7:]
+            col = (0, 1) if name.startswith("occupied_source") else (2, 3)
+            count = int(np.count_nonzero(leaves[:, col[1]] - leaves[:, col[0]]))
+            return self.tree_ints(name, count)
+        size = self.lib.fn("fmm_array_size")(self.handle, name.encode())
+        if size < 0:
+            raise KeyError(name)
+        return self.tree_ints(name, size)
+
+    def max_level(self) -> int:
+        nn = self.sizes()["nodes"]
+        level, total = 0, 1
+        while total < nn:
+            level += 1
+            total += 8 ** level
+        return level
+
+    def coefficients(self, which: str, node: int) -> np.ndarray:
+        out = np.zeros(self.sizes()["basis"])
+        self.lib.check(self.lib.fn("fmm_coefficients")(self.handle, 0 if which == "multipole" else 1, node, _dp(out)))
+        return out
+
+
+def m2l_matrix(order: int, offset, kind: str = "oracle") -> np.ndarray:
+    """Column-major class matrix, returned as T[beta, alpha]."""
+    lib = library(kind)
+    n = (order + 1) * (order + 2) * (order + 3) // 6
+    out = np.zeros(n * n)
+    lib.check(lib.fn("m2l_matrix")(order, *[float(v) for v in offset], _dp(out)))
+    return out.reshape(n, n).T.copy()
+
+
+def laplace_derivatives(order: int, r, kind: str = "oracle") -> np.ndarray:
+    lib = library(kind)
+    n = (order + 1) * (order + 2) * (order + 3) // 6
+    out = np.zeros(n)
+    lib.check(lib.fn("laplace_derivatives")(order, *[float(v) for v in r], _dp(out)))
+    return out
+
+
+def direct(targets, sources, moments, ids=None, output: str = "field", kind: str = "oracle"):
+    """FP64 O(N_t N_s) direct sum with self exclusion (validation.cpp:71-111)."""
+    lib = library(kind)
+    flags = {"potential": 1, "field": 2, "both": 3}[output]
+    t = np.ascontiguousarray(targets, dtype=np.float64)
+    s = np.ascontiguousarray(sources, dtype=np.float64)
+    m = np.ascontiguousarray(moments, dtype=np.float64)
+    H = np.zeros((len(t), 3))
+    phi = np.zeros(len(t))
+    idv = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+    lib.check(lib.fn("direct")(_dp(t), len(t), _dp(s), len(s), _dp(m),
+                               None if idv is None else idv.ctypes.data_as(_IP), flags, _dp(H), _dp(phi)))
+    return {"H": H, "phi": phi}
+
+
+def basis(order: int) -> np.ndarray:
+    """Multi-index basis in storage order (include/cdfmm/multi_index.hpp:62-71)."""
+    out = []
+    for d in range(order + 1):
+        for ax in range(d + 1):
+            for ay in range(d - ax + 1):
+                out.append((ax, ay, d - ax - ay))
+    return np.array(out, dtype=np.int64).reshape(-1, 3)

@@ -1,0 +1,267 @@
+"""HipFull device path: parity with the reference CpuStatic path (needs one MI355X).
+
+Bars (north star): FP32 plans within rel. L2 1e-5 of CpuStatic in the same
+precision, FP64 plans within 1e-12; the MFMA split-precision M2L within 1e-5
+of the FP64 FMM at the same (p, depth).  The expected values come from the
+committed golden fixtures written by the compiled reference and from the
+oracle restatement (oracle/cpustatic.cpp) on the same seeded inputs.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_l2
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import cdfmm_hip
+
+    assert cdfmm_hip.hip_full_available(), "no gfx950 device is visible to the HIP runtime"
+    return cdfmm_hip
+
+
+def make(hip, src, tgt, p, depth, prec=32, ids=None, mode=None, root=None):
+    o = hip.UniformFmmOptions()
+    o.expansion_order = p
+    o.tree.max_level = depth
+    o.backend = hip.ExecutionBackend.HIP_FULL
+    o.precision = hip.StaticPrecision.FLOAT32 if prec == 32 else hip.StaticPrecision.FLOAT64
+    if ids is not None:
+        o.fixed_target_source_indices = [int(v) for v in ids]
+    if mode is not None:
+        o.hip_m2l_mode = mode
+    if root is not None:
+        o.tree.root_centre = hip.Vec3(*[float(v) for v in root[:3]])
+        o.tree.root_half_width = float(root[3])
+    return hip.UniformFmm(src, tgt, o)
+
+
+def announce(*what):
+    print("case:", *what, flush=True)
+
+
+GOLDEN_CASES = ["c1", "n4096_p4_d3", "n4096_p4_d4", "n4096_p6_d3", "n4096_p6_d4", "n4096_p8_d3", "n4096_p8_d4"]
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_fields_match_reference_golden(hip, case):
+    g = golden("ref_fields.npz")
+    x, m = g[f"{case}/positions"], g[f"{case}/moments"]
+    n, p, d = (int(v) for v in g[f"{case}/config"])
+    ids = np.arange(n)
+    for prec, tol in ((32, 1e-5), (64, 1e-12)):
+        key = f"{case}/H{prec}"
+        if key not in g.files:
+            continue
+        announce("golden", case, prec)
+        f = make(hip, x, x, p, d, prec, ids=ids)
+        h = f.evaluate(m, target_source_indices=ids)["H"]
+        assert h.dtype == (np.float32 if prec == 32 else np.float64)
+        err = rel_l2(h, g[key])
+        print(f"  rel L2 vs reference CpuStatic: {err:.3e}", flush=True)
+        assert err <= tol
+
+
+def test_separate_targets_without_identities(hip):
+    g = golden("ref_fields.npz")
+    for prec, tol in ((32, 1e-5), (64, 1e-12)):
+        announce("separate", prec)
+        f = make(hip, g["separate/sources"], g["separate/targets"], 4, 3, prec, root=(0, 0, 0, 1.0))
+        assert rel_l2(f.evaluate(g["separate/moments"])["H"], g[f"separate/H{prec}"]) <= tol
+
+
+@pytest.mark.parametrize("p,depth", [(1, 2), (3, 2), (5, 3), (7, 3)])
+def test_matches_oracle_both_precisions(hip, p, depth):
+    rng = np.random.default_rng(100 + p)
+    x = rng.uniform(-0.9, 0.9, size=(3000, 3))
+    m = rng.normal(size=(3000, 3))
+    ids = np.arange(3000)
+    for prec, tol in ((32, 1e-5), (64, 1e-12)):
+        announce("oracle", p, depth, prec)
+        h = make(hip, x, x, p, depth, prec, ids=ids).evaluate(m)["H"]
+        ref = O.CpuFmm(x, x, p, depth, prec, fixed_ids=ids).evaluate(m)["H"]
+        err = rel_l2(h, ref)
+        print(f"  rel L2 vs CpuStatic restatement: {err:.3e}", flush=True)
+        assert err <= tol
+
+
+@pytest.mark.parametrize("depth", [0, 1])
+def test_shallow_trees_are_pure_near_field(hip, depth):
+    rng = np.random.default_rng(depth)
+    x = rng.uniform(0, 1, size=(700, 3))
+    m = rng.normal(size=(700, 3))
+    ids = np.arange(700)
+    announce("shallow", depth)
+    h = make(hip, x, x, 3, depth, 64, ids=ids).evaluate(m)["H"]
+    d = O.direct(x, x, m, ids)["H"]
+    assert rel_l2(h, d) <= 1e-13
+
+
+def test_expansions_match_oracle(hip):
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 1, size=(4000, 3))
+    m = rng.normal(size=(4000, 3))
+    ids = np.arange(4000)
+    announce("expansions")
+    f = make(hip, x, x, 4, 3, 64, ids=ids)
+    f.evaluate(m)
+    o = O.CpuFmm(x, x, 4, 3, 64, fixed_ids=ids)
+    o.evaluate(m)
+    for node in list(range(9, 73, 7)) + list(range(73, 585, 41)):
+        np.testing.assert_allclose(f.multipole(node), o.coefficients("multipole", node), rtol=1e-11, atol=1e-11)
+    for node in range(73, 585, 41):
+        ref = o.coefficients("local", node)
+        np.testing.assert_allclose(f.local(node), ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("prec,K", [(32, 8), (32, 32), (64, 5)])
+def test_batch_equals_independent_evaluates(hip, prec, K):
+    rng = np.random.default_rng(K)
+    x = rng.uniform(0, 1, size=(4096, 3))
+    ms = rng.normal(size=(K, 4096, 3))
+    ids = np.arange(4096)
+    announce("batch", prec, K)
+    f = make(hip, x, x, 4, 3, prec, ids=ids)
+    batch = f.evaluate_batch(ms)
+    assert batch.shape == (K, 4096, 3)
+    tol = 2e-6 if prec == 32 else 1e-13
+    for k in range(K):
+        single = f.evaluate(ms[k])["H"]
+        assert rel_l2(batch[k], single) <= tol, k
+
+
+def test_repeated_evaluates_are_bitwise_identical(hip):
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0, 1, size=(5000, 3))
+    m = rng.normal(size=(5000, 3))
+    announce("determinism")
+    f = make(hip, x, x, 4, 3, 32, ids=np.arange(5000))
+    a = f.evaluate(m)["H"]
+    b = f.evaluate(m)["H"]
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("p,depth,n", [(2, 2, 800), (4, 3, 8000), (6, 3, 20000), (8, 3, 20000)])
+def test_mfma_bf16x3_within_bar_of_fp64_fmm(hip, p, depth, n):
+    rng = np.random.default_rng(300 + p)
+    x = rng.uniform(0, 1, size=(n, 3))
+    m = rng.normal(size=(n, 3))
+    ids = np.arange(n)
+    announce("mfma", p, depth, n)
+    f = make(hip, x, x, p, depth, 32, ids=ids, mode=hip.HipM2LMode.MFMA_BF16X3)
+    h_mfma = f.evaluate(m)["H"]
+    f.set_m2l_mode(hip.HipM2LMode.VALU)
+    h_valu = f.evaluate(m)["H"]
+    ref64 = O.CpuFmm(x, x, p, depth, 64, fixed_ids=ids).evaluate(m)["H"]
+    e_mfma, e_valu = rel_l2(h_mfma, ref64), rel_l2(h_valu, ref64)
+    print(f"  rel L2 vs FP64 FMM: mfma {e_mfma:.3e}  valu {e_valu:.3e}", flush=True)
+    assert e_mfma <= 1e-5
+
+
+def test_identity_map_contract(hip):
+    rng = np.random.default_rng(2)
+    x = rng.uniform(0, 1, size=(500, 3))
+    m = rng.normal(size=(500, 3))
+    ids = np.arange(500)
+    announce("identity contract")
+    f = make(hip, x, x, 3, 2, 32)
+    f.evaluate(m, target_source_indices=ids)
+    with pytest.raises(ValueError, match="identity map changed"):
+        f.evaluate(m, target_source_indices=ids[::-1].copy())
+    g = make(hip, x, x, 3, 2, 32, ids=ids)
+    np.testing.assert_array_equal(g.evaluate(m)["H"], g.evaluate(m, target_source_indices=ids)["H"])
+    with pytest.raises(ValueError, match="identity map changed"):
+        g.evaluate(m, target_source_indices=np.full(500, -1))
+    with pytest.raises(ValueError, match="field-only"):
+        g.evaluate(m, output="both")
+    with pytest.raises(ValueError, match="identity map contains an invalid index"):
+        g.evaluate(m[:-1])
+    with pytest.raises(ValueError, match="dimensions are inconsistent"):
+        f.evaluate(m[:-1])
+
+
+def test_coincident_non_identity_pair_gives_nan(hip):
+    x = np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3], [0.8, 0.7, 0.6], [0.4, 0.5, 0.9]])
+    m = np.eye(4, 3) + 0.1
+    announce("coincident")
+    for prec in (32, 64):
+        h = make(hip, x, x, 2, 1, prec).evaluate(m, target_source_indices=[0, -1, 2, 3])["H"]
+        assert np.isnan(h[0]).all() and np.isnan(h[1]).all()
+        assert np.isfinite(h[2]).all() and np.isfinite(h[3]).all()
+
+
+def test_transfer_and_upload_contract(hip):
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 1, size=(2000, 3))
+    t = rng.uniform(0, 1, size=(1500, 3))
+    announce("statistics")
+    f = make(hip, x, t, 4, 3, 32)
+    s0 = f.hip_plan_statistics
+    f.evaluate(rng.normal(size=(2000, 3)))
+    s1 = f.hip_plan_statistics
+    print("  statistics:", s1, flush=True)
+    assert s1["evaluation_h2d_bytes"] == 2000 * 12 and s1["evaluation_d2h_bytes"] == 1500 * 12
+    assert s1["evaluation_h2d_calls"] == s0["evaluation_h2d_calls"] + 1
+    assert s1["evaluation_d2h_calls"] == s0["evaluation_d2h_calls"] + 1
+    assert s1["static_upload_count"] == 1 and s1["plan_generation_count"] == 1
+    assert s1["m2l_unique_matrix_count"] == 316 and s1["scalar_bytes"] == 4
+    ref = O.CpuFmm(x, t, 4, 3, 32)
+    assert s1["m2l_interaction_count"] == len(ref.array("m2l_sources"))
+    assert s1["p2p_interaction_count"] == int(ref.array("p2p_rows")[-1])
+    tm = f.last_timings
+    assert tm["evaluations"] == 1 and tm["total"] > 0 and tm["m2l"] > 0
+
+
+def test_empty_geometry(hip):
+    announce("empty")
+    e = np.zeros((0, 3))
+    for prec in (32, 64):
+        f = make(hip, e, e, 3, 2, prec)
+        assert f.evaluate(e)["H"].shape == (0, 3)
+    x = np.random.default_rng(1).uniform(size=(50, 3))
+    f = make(hip, x, e, 3, 2, 32)
+    assert f.evaluate(np.ones((50, 3)))["H"].shape == (0, 3)
+
+
+def test_device_resident_evaluate_matches_host(hip):
+    import torch
+
+    rng = np.random.default_rng(21)
+    x = rng.uniform(0, 1, size=(6000, 3))
+    K = 4
+    ms = rng.normal(size=(K, 6000, 3))
+    ids = np.arange(6000)
+    announce("device-resident")
+    f = make(hip, x, x, 4, 3, 32, ids=ids)
+    host = f.evaluate_batch(ms)
+    dev_m = torch.tensor(ms, dtype=torch.float32, device="cuda:0")
+    dev_h = torch.empty((K, 6000, 3), dtype=torch.float32, device="cuda:0")
+    f.evaluate_device(dev_m.data_ptr(), dev_h.data_ptr(), K, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert rel_l2(dev_h.cpu().numpy(), host) <= 1e-6
+
+
+def test_gpu_direct_sum_matches_oracle():
+    import ctypes
+    import os
+
+    from conftest import REPO
+
+    rng = np.random.default_rng(6)
+    s = rng.uniform(0, 1, size=(5000, 3))
+    m = rng.normal(size=(5000, 3))
+    t = s[:700].copy()
+    ids = np.arange(700, dtype=np.int32)
+    announce("gpu direct")
+    lib = ctypes.CDLL(os.path.join(REPO, "dip-fmm_amd", "libhipfmm.so"))
+    out = np.zeros((700, 3))
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib.hipfmm_direct_field(t.ctypes.data_as(dp), 700, s.ctypes.data_as(dp), 5000, m.ctypes.data_as(dp),
+                                 ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), out.ctypes.data_as(dp))
+    assert rc == 0
+    assert rel_l2(out, O.direct(t, s, m, ids)["H"]) <= 1e-13

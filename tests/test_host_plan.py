@@ -1,0 +1,148 @@
+"""HipFull host side and boundary, CPU only.
+
+* The host octree of the product (cdfmm_hip.UniformTree) reproduces the
+  reference plan bit for bit: Morton permutations, node ranges, list1/list2
+  (the lists are enumerated from box offsets exactly as the device kernels
+  enumerate them).
+* The host FP64 translation operators (Laplace derivatives by recurrence,
+  M2L class matrices) agree with the oracle's Taylor jets.
+* libhipfmm.so loads and exports every entry point include/hipfmm.h declares;
+  without a device the front-end fails as the reference's does for CudaFull.
+"""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO, golden
+
+import oracle as O
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import cdfmm_hip
+
+    return cdfmm_hip
+
+
+def tree_options(hip, depth, root=None):
+    o = hip.UniformTreeOptions()
+    o.max_level = depth
+    if root is not None:
+        o.root_centre = hip.Vec3(*[float(v) for v in root[:3]])
+        o.root_half_width = float(root[3])
+    return o
+
+
+def product_tree_arrays(tree):
+    ranges = np.asarray(tree.node_ranges)
+    nn = tree.node_count
+    out = {
+        "source_permutation": np.asarray(tree.source_permutation),
+        "target_permutation": np.asarray(tree.target_permutation),
+        "node_ranges": ranges.reshape(-1),
+    }
+    for name in ("list1", "list2"):
+        lists = [np.asarray(getattr(tree, name)(k)) for k in range(nn)]
+        offs = np.concatenate([[0], np.cumsum([len(v) for v in lists])])
+        out[name] = np.concatenate([offs, np.concatenate(lists) if lists else np.zeros(0)]).astype(np.int64)
+    return out
+
+
+def test_host_tree_matches_golden_edge_cases(hip):
+    g = golden("ref_trees.npz")
+    for name in sorted({k.split("/")[0] for k in g.files}):
+        root = g[f"{name}/fixed_root"] if f"{name}/fixed_root" in g.files else None
+        t = hip.UniformTree(g[f"{name}/sources"], g[f"{name}/targets"],
+                            tree_options(hip, int(g[f"{name}/depth"]), root))
+        assert (t.root_centre.x, t.root_centre.y, t.root_centre.z, t.root_half_width) == tuple(g[f"{name}/root"])
+        arrays = product_tree_arrays(t)
+        for a, v in arrays.items():
+            np.testing.assert_array_equal(v, g[f"{name}/{a}"], err_msg=f"{name}/{a}")
+
+
+@pytest.mark.parametrize("depth,n", [(3, 4000), (4, 20000)])
+def test_host_tree_matches_oracle_plan(hip, depth, n):
+    rng = np.random.default_rng(depth)
+    x = rng.uniform(0, 1, size=(n, 3))
+    t = hip.UniformTree(x, x, tree_options(hip, depth))
+    f = O.CpuFmm(x, x, 1, depth, 64)
+    arrays = product_tree_arrays(t)
+    for a in ("source_permutation", "target_permutation", "node_ranges", "list1", "list2"):
+        np.testing.assert_array_equal(arrays[a], f.array(a), err_msg=a)
+    np.testing.assert_array_equal(f.root(), [t.root_centre.x, t.root_centre.y, t.root_centre.z, t.root_half_width])
+    # Node centres follow the reference formula exactly (used for P2M/L2P offsets).
+    c = [t.node_centre(k) for k in (0, 9, 72, 584)]
+    assert all(np.all(np.isfinite(v)) for v in c)
+
+
+def test_host_laplace_recurrence_matches_taylor_jets(hip):
+    rng = np.random.default_rng(17)
+    for order in (4, 8, 12, 16):
+        for _ in range(3):
+            r = rng.integers(-3, 4, size=3).astype(float)
+            if np.max(np.abs(r)) < 2:
+                r[0] = 2.0
+            a = hip._laplace_derivatives(order, tuple(r))
+            b = O.laplace_derivatives(order, tuple(r))
+            scale = np.abs(b).max()
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-14 * scale)
+
+
+def test_host_m2l_matrices_match_golden(hip):
+    g = golden("ref_m2l.npz")
+    for key in g.files:
+        if "/matrix_" not in key:
+            continue
+        p = int(key[1])
+        off = tuple(int(v) for v in key.split("matrix_")[1].split("_"))
+        ref = g[key]
+        mine = hip._m2l_class_matrix(p, off)
+        np.testing.assert_allclose(mine, ref, rtol=2e-13, atol=1e-15 * np.abs(ref).max())
+        # FP32 copies as uploaded: within one float ulp of the reference's
+        # quantised values wherever an entry matters at FP32 resolution.
+        a32, b32 = mine.astype(np.float32), ref.astype(np.float32)
+        big = np.abs(ref) > 1e-6 * np.abs(ref).max()
+        assert np.all(np.abs(a32 - b32)[big] <= np.spacing(np.abs(b32))[big])
+
+
+def header_functions():
+    text = open(os.path.join(REPO, "include", "hipfmm.h")).read()
+    return sorted(set(re.findall(r"\b(hipfmm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_c_abi_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(os.path.join(REPO, "dip-fmm_amd", "libhipfmm.so"))
+    names = header_functions()
+    assert len(names) >= 11
+    for name in names:
+        assert hasattr(lib, name), name
+    lib.hipfmm_runtime_available.restype = ctypes.c_int
+    assert lib.hipfmm_runtime_available() in (0, 1)
+    lib.hipfmm_plan_destroy(None)  # documented no-op
+
+
+def test_front_end_errors_follow_the_reference(hip):
+    x = np.random.default_rng(1).uniform(size=(10, 3))
+    opts = hip.UniformFmmOptions()
+    opts.expansion_order = -1
+    opts.backend = hip.ExecutionBackend.HIP_FULL
+    with pytest.raises(ValueError, match="expansion_order must be >= 0"):
+        hip.UniformFmm(x, x, opts)
+    opts = hip.UniformFmmOptions()
+    with pytest.raises(RuntimeError, match="CpuStatic is provided by the reference library"):
+        hip.UniformFmm(x, x, opts)
+    assert int(hip.ExecutionBackend.HIP_FULL) != int(hip.ExecutionBackend.CUDA_FULL)
+    if not hip.hip_full_available():
+        opts.backend = hip.ExecutionBackend.HIP_FULL
+        with pytest.raises(RuntimeError, match="HipFull is unavailable in this build"):
+            hip.UniformFmm(x, x, opts)
+    opts.tree.max_level = 1
+    with pytest.raises(ValueError, match="outside the requested root box"):
+        opts.tree.root_centre = hip.Vec3(0.0, 0.0, 0.0)
+        opts.tree.root_half_width = 0.1
+        hip.UniformFmm(x, x, opts)

@@ -1,0 +1,221 @@
+"""Pins of the oracle (oracle/cpustatic.cpp), CPU only.
+
+The oracle restates the reference CpuStatic path.  It is checked here against
+* the committed golden fixtures written by the compiled reference
+  (tests/golden/make_golden.py),
+* the compiled reference itself where it is built (oracle/_ref),
+* closed forms: on-axis Laplace derivatives and dipole fields, the Laplace
+  trace identity, homogeneity and symmetry of the M2L class matrices, and
+  the depth-0 FMM, which equals the direct sum.
+Each check is chosen so that a dropped term, a wrong sign, a wrong index or a
+transposed operand fails at least one of them.
+"""
+
+import itertools
+import hashlib
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_l2
+
+import oracle as O
+
+C = 1.0 / (4.0 * math.pi)
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference library not built here")
+
+
+def test_basis_sizes_and_order():
+    # test_multi_index.cpp:7-13 sizes, multi_index.hpp:62-71 order.
+    assert [len(O.basis(p)) for p in (0, 1, 2, 3, 5)] == [1, 4, 10, 20, 56]
+    b = O.basis(2)
+    assert b.tolist() == [[0, 0, 0], [0, 0, 1], [0, 1, 0], [1, 0, 0], [0, 0, 2], [0, 1, 1], [0, 2, 0],
+                          [1, 0, 1], [1, 1, 0], [2, 0, 0]]
+
+
+def test_laplace_derivatives_on_axis_closed_form():
+    # test_laplace_derivatives.cpp:11-25: at r = (x, 0, 0)
+    x = 1.7
+    d = O.laplace_derivatives(2, (x, 0.0, 0.0))
+    idx = {tuple(m): i for i, m in enumerate(O.basis(2).tolist())}
+    assert d[idx[(0, 0, 0)]] == pytest.approx(C / x, rel=1e-15)
+    assert d[idx[(1, 0, 0)]] == pytest.approx(-C / x**2, rel=1e-15)
+    assert d[idx[(2, 0, 0)]] == pytest.approx(2 * C / x**3, rel=1e-15)
+    assert d[idx[(0, 2, 0)]] == pytest.approx(-C / x**3, rel=1e-15)
+    assert d[idx[(0, 0, 2)]] == pytest.approx(-C / x**3, rel=1e-15)
+    assert abs(d[idx[(0, 1, 0)]]) < 1e-18 and abs(d[idx[(1, 1, 0)]]) < 1e-18
+
+
+def test_laplace_derivatives_against_closed_form_high_order():
+    # d^k/dx^k (1/x) = (-1)^k k! / x^(k+1) on the x axis, every order to 16.
+    x = 2.3
+    q = 16
+    d = O.laplace_derivatives(q, (x, 0.0, 0.0))
+    idx = {tuple(m): i for i, m in enumerate(O.basis(q).tolist())}
+    for k in range(q + 1):
+        expect = C * (-1) ** k * math.factorial(k) / x ** (k + 1)
+        assert d[idx[(k, 0, 0)]] == pytest.approx(expect, rel=1e-12)
+
+
+def test_laplace_trace_vanishes():
+    # Harmonicity: sum_i D_{g+2e_i} G = 0 for every g (test_laplace_derivatives.cpp:27-42).
+    q = 10
+    r = (1.3, -0.7, 2.1)
+    d = O.laplace_derivatives(q, r)
+    basis = O.basis(q).tolist()
+    idx = {tuple(m): i for i, m in enumerate(basis)}
+    for g in basis:
+        if sum(g) + 2 > q:
+            continue
+        t = sum(d[idx[(g[0] + 2 * (a == 0), g[1] + 2 * (a == 1), g[2] + 2 * (a == 2))]] for a in range(3))
+        scale = max(abs(d[idx[(g[0] + 2 * (a == 0), g[1] + 2 * (a == 1), g[2] + 2 * (a == 2))]]) for a in range(3))
+        assert abs(t) <= 1e-11 * scale
+
+
+def test_m2l_matrix_homogeneity_and_symmetry():
+    # T(h t) = h^-(|a|+|b|+1) T(t) (test_static_m2l.cpp:357-379); T[b][a]
+    # depends on a+b only, so T is symmetric.
+    p = 4
+    b = O.basis(p)
+    deg = b.sum(axis=1)
+    t1 = O.m2l_matrix(p, (2, -1, 3))
+    t2 = O.m2l_matrix(p, (4, -2, 6))
+    scale = 2.0 ** (-(deg[:, None] + deg[None, :] + 1))
+    np.testing.assert_allclose(t2, scale * t1, rtol=1e-13, atol=0)
+    np.testing.assert_array_equal(t1, t1.T)
+
+
+def test_m2l_matrices_match_golden_digests():
+    g = golden("ref_m2l.npz")
+    for p in (4, 6):
+        for off, digest in zip(g[f"p{p}/offsets"], g[f"p{p}/sha256"]):
+            mat = O.m2l_matrix(p, tuple(int(v) for v in off))
+            assert hashlib.sha256(np.ascontiguousarray(mat).tobytes()).hexdigest() == digest, (p, off)
+    # p = 8 (order-16 jets): full matrices of three offsets.
+    for key in g.files:
+        if key.startswith("p8/matrix_"):
+            off = tuple(int(v) for v in key.split("_")[1:])
+            np.testing.assert_array_equal(O.m2l_matrix(8, off), g[key])
+
+
+def test_direct_sum_closed_forms():
+    # test_p2p.cpp:13-33: axial H = 2c x, transverse H = -c x.
+    s = np.zeros((1, 3))
+    m = np.array([[1.0, 0.0, 0.0]])
+    t = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    r = O.direct(t, s, m)
+    np.testing.assert_allclose(r["H"], [[2 * C, 0, 0], [-C, 0, 0]], rtol=1e-15, atol=1e-18)
+
+
+def test_tree_matches_golden_edge_cases():
+    g = golden("ref_trees.npz")
+    names = sorted({k.split("/")[0] for k in g.files})
+    for name in names:
+        kw = {}
+        if f"{name}/fixed_root" in g.files:
+            fr = g[f"{name}/fixed_root"]
+            kw = {"root_centre": fr[:3], "root_half_width": fr[3]}
+        f = O.CpuFmm(g[f"{name}/sources"], g[f"{name}/targets"], 2, int(g[f"{name}/depth"]), 64, **kw)
+        np.testing.assert_array_equal(f.root(), g[f"{name}/root"])
+        for a in ("source_permutation", "target_permutation", "node_ranges", "list1", "list2"):
+            np.testing.assert_array_equal(f.array(a), g[f"{name}/{a}"], err_msg=f"{name}/{a}")
+
+
+@pytest.mark.parametrize("case", ["c1", "n4096_p4_d3", "n4096_p4_d4", "n4096_p6_d3", "n4096_p6_d4",
+                                  "n4096_p8_d3", "n4096_p8_d4"])
+def test_fields_match_golden(case):
+    g = golden("ref_fields.npz")
+    x, m = g[f"{case}/positions"], g[f"{case}/moments"]
+    n, p, d = (int(v) for v in g[f"{case}/config"])
+    ids = np.arange(n, dtype=np.int32)
+    for prec, tol in ((64, 1e-13), (32, 1e-6)):
+        key = f"{case}/H{prec}"
+        if key not in g.files:
+            continue
+        r = O.CpuFmm(x, x, p, d, prec, fixed_ids=ids).evaluate(m, ids)
+        assert rel_l2(r["H"], g[key]) <= tol, (case, prec)
+
+
+def test_separate_targets_match_golden():
+    g = golden("ref_fields.npz")
+    for prec, tol in ((64, 1e-13), (32, 1e-6)):
+        f = O.CpuFmm(g["separate/sources"], g["separate/targets"], 4, 3, prec, root_centre=(0, 0, 0),
+                     root_half_width=1.0)
+        assert rel_l2(f.evaluate(g["separate/moments"])["H"], g[f"separate/H{prec}"]) <= tol
+
+
+def test_depth_zero_equals_direct_sum():
+    # test_uniform_fmm.cpp:429-451: with one leaf every pair is list-1 P2P.
+    rng = np.random.default_rng(9281)
+    x = rng.uniform(-1, 1, size=(300, 3))
+    m = rng.normal(size=(300, 3))
+    ids = np.arange(300)
+    f = O.CpuFmm(x, x, 3, 0, 64)
+    r = f.evaluate(m, ids, output="both")
+    d = O.direct(x, x, m, ids, output="both")
+    np.testing.assert_allclose(r["H"], d["H"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(r["phi"], d["phi"], rtol=1e-12, atol=1e-12)
+
+
+def test_fmm_converges_to_direct_sum():
+    # test_uniform_fmm.cpp:380-427: p=4 relative RMS < 2e-2; error falls with p.
+    rng = np.random.default_rng(741)
+    x = rng.uniform(0, 1, size=(3000, 3))
+    m = rng.normal(size=(3000, 3))
+    ids = np.arange(3000)
+    d = O.direct(x, x, m, ids)["H"]
+    errs = []
+    for p in (2, 4, 6):
+        h = O.CpuFmm(x, x, p, 3, 64, fixed_ids=ids).evaluate(m)["H"]
+        errs.append(rel_l2(h, d))
+    assert errs[1] < 2e-2 and errs[2] < errs[1] < errs[0]
+
+
+def test_coincident_non_identity_pair_is_nan():
+    # static_operators.cpp:435-446 / test_static_m2l.cpp:146-167.
+    x = np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3], [0.8, 0.7, 0.6]])
+    m = np.eye(3)
+    for prec in (64, 32):
+        f = O.CpuFmm(x, x, 2, 1, prec)
+        h = f.evaluate(m, np.array([0, -1, 2]))["H"]
+        assert np.isnan(h[1]).all() and np.isnan(h[0]).all() and np.isfinite(h[2]).all()
+
+
+def test_identity_map_errors():
+    x = np.random.default_rng(3).uniform(size=(20, 3))
+    ids = np.arange(20)
+    f = O.CpuFmm(x, x, 2, 1, 32, fixed_ids=ids)
+    with pytest.raises(ValueError, match="identity map changed"):
+        f.evaluate(np.ones((20, 3)), ids[::-1].copy())
+    with pytest.raises(ValueError, match="identity map contains an invalid index"):
+        f.evaluate(np.ones((19, 3)))
+    g = O.CpuFmm(x, x, 2, 1, 32)
+    with pytest.raises(ValueError, match="one dipole moment per source"):
+        g.evaluate(np.ones((19, 3)))
+
+
+@needs_ref
+@pytest.mark.parametrize("prec", [64, 32])
+def test_plan_and_expansions_bit_exact_against_reference(prec):
+    rng = np.random.default_rng(8123)
+    x = rng.uniform(0, 1, size=(2500, 3))
+    m = rng.normal(size=(2500, 3))
+    ids = np.arange(2500)
+    a = O.CpuFmm(x, x, 4, 3, prec, fixed_ids=ids)
+    b = O.CpuFmm(x, x, 4, 3, prec, fixed_ids=ids, kind="ref")
+    for name in ("source_permutation", "target_permutation", "node_ranges", "list1", "list2",
+                 "occupied_source_leaves", "occupied_target_leaves"):
+        np.testing.assert_array_equal(a.array(name), b.array(name), err_msg=name)
+    ra, rb = a.evaluate(m), b.evaluate(m)
+    for node in itertools.chain(range(0, 73), range(73, 585, 37)):
+        np.testing.assert_array_equal(a.coefficients("multipole", node), b.coefficients("multipole", node))
+        np.testing.assert_array_equal(a.coefficients("local", node), b.coefficients("local", node))
+    assert rel_l2(ra["H"], rb["H"]) <= (1e-15 if prec == 64 else 1e-7)
+
+
+@needs_ref
+def test_m2l_matrices_bit_exact_against_reference():
+    for p in (2, 4, 6):
+        for off in ((2, 0, 0), (3, -2, 1), (-3, -3, -3), (2, 3, -1)):
+            np.testing.assert_array_equal(O.m2l_matrix(p, off), O.m2l_matrix(p, off, kind="ref"))
