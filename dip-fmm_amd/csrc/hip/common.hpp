@@ -106,6 +106,7 @@ struct DevicePlan {
   // operators
   void* m2l_T = nullptr;     // FP32/FP64 class matrices, column-major [class][alpha][beta]
   int* m2l_slot = nullptr;   // 343 slot -> class index or -1
+  int4* parity_offsets = nullptr;  // [8 parities][189] (ox, oy, oz, class) list-2 offsets
   void* ms = nullptr;        // (L+1) x n
   void* ls = nullptr;
   ShiftCsr m2m, l2l;
@@ -137,7 +138,7 @@ void launch_m2m(const DevicePlan& d, int parent_level, int K, hipStream_t s);
 template <class S>
 void launch_l2l(const DevicePlan& d, int child_level, int K, hipStream_t s);
 template <class S>
-void launch_m2l_valu(const DevicePlan& d, int level, int K, hipStream_t s);
+void launch_m2l_valu(const DevicePlan& d, int K, hipStream_t s);  // all levels 2..L
 template <class S>
 void launch_p2p(const DevicePlan& d, int K, hipStream_t s);
 template <class S>
@@ -145,8 +146,8 @@ void launch_l2p_unsort(const DevicePlan& d, S* fields, int K, hipStream_t s);
 template <class S>
 void launch_zero(S* ptr, size_t count, hipStream_t s);
 
-void launch_m2l_split(const DevicePlan& d, int level, int K, int mode, hipStream_t s);
-void launch_m2l_mfma(const DevicePlan& d, int level, int K, int mode, hipStream_t s);
+void launch_m2l_split(const DevicePlan& d, int K, int mode, hipStream_t s);  // all levels 2..L
+void launch_m2l_mfma(const DevicePlan& d, int K, int mode, hipStream_t s);   // all levels 2..L
 void prepare_mfma_operators(DevicePlan& d, const std::vector<double>& matrices, int classes,
                             const std::vector<int>& slot_of_class, const std::vector<double>& local_scaling,
                             std::vector<void*>& allocations, size_t& bytes);

@@ -15,7 +15,8 @@
 // LDS.  Each thread owns one output coefficient b for all 16 columns
 // (column = target box x moment state); accumulation is in registers and
 // every local is written once, without atomics.  Empty source boxes carry
-// exact zero multipoles, so they need no test.
+// exact zero multipoles, so they need no test.  One launch covers every
+// level 2..L, so the small upper levels overlap the leaf level.
 #include "common.hpp"
 
 namespace hipfmm {
@@ -24,23 +25,38 @@ namespace {
 
 constexpr int kCols = 16;
 
+struct ValuLevels {
+  int count;
+  int block_begin[10];
+  int tiles[10];  // tiles per parity
+  int boxes_per_tile;
+  int k_per_tile;
+  int kchunks;
+};
+
 template <class S>
-__global__ void m2l_valu_kernel(const DevicePlan d, int level, int K, int boxes_per_tile, int k_per_tile) {
+__global__ void m2l_valu_kernel(const DevicePlan d, const ValuLevels lv, int K) {
   extern __shared__ unsigned char smem_raw[];
   S* ms_tile = reinterpret_cast<S*>(smem_raw);  // n x kCols
   __shared__ int4 tbox[kCols];                   // target box coords + valid flag
   const int n = d.n;
-  const int parity = blockIdx.y;
+  int li = lv.count - 1;
+  while (li > 0 && int(blockIdx.x) < lv.block_begin[li]) --li;
+  const int level = 2 + li;
+  const int lb = int(blockIdx.x) - lv.block_begin[li];
+  const int parity = lb & 7, rest = lb >> 3;
+  const int tile = rest % lv.tiles[li], kc = rest / lv.tiles[li];
   const int px = parity & 1, py = (parity >> 1) & 1, pz = (parity >> 2) & 1;
   const int nbox = 1 << level;
   const int64_t parents = int64_t(1) << (3 * (level - 1));
-  const int64_t q0 = int64_t(blockIdx.x) * boxes_per_tile;
-  const int k0 = blockIdx.z * k_per_tile;
+  const int bpt = lv.boxes_per_tile, kpt = lv.k_per_tile;
+  const int64_t q0 = int64_t(tile) * bpt;
+  const int k0 = kc * kpt;
   const int64_t lvl0 = level_offset(level);
   if (threadIdx.x < kCols) {
-    const int b = threadIdx.x / k_per_tile;
+    const int b = threadIdx.x / kpt;
     int4 tb{0, 0, 0, 0};
-    if (b < boxes_per_tile && q0 + b < parents && k0 + threadIdx.x % k_per_tile < K) {
+    if (b < bpt && q0 + b < parents && k0 + threadIdx.x % kpt < K) {
       int x, y, z;
       demorton3(uint32_t(q0 + b), level - 1, x, y, z);
       tb = int4{2 * x + px, 2 * y + py, 2 * z + pz, 1};
@@ -50,38 +66,37 @@ __global__ void m2l_valu_kernel(const DevicePlan d, int level, int K, int boxes_
   const S* M = static_cast<const S*>(d.M);
   const S* msl = static_cast<const S*>(d.ms) + int64_t(level) * n;
   const S* T = static_cast<const S*>(d.m2l_T);
+  const int4* offs = d.parity_offsets + parity * 189;
   const int beta = threadIdx.x;
   S acc[kCols];
 #pragma unroll
   for (int c = 0; c < kCols; ++c) acc[c] = S(0);
-  for (int ox = px - 3; ox <= px + 2; ++ox)
-    for (int oy = py - 3; oy <= py + 2; ++oy)
-      for (int oz = pz - 3; oz <= pz + 2; ++oz) {
-        if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
-        const int cls = d.m2l_slot[offset_slot(ox, oy, oz)];
-        __syncthreads();
-        for (int w = threadIdx.x; w < n * kCols; w += blockDim.x) {
-          const int a = w / kCols, col = w % kCols;
-          const int4 tb = tbox[col];
-          const int sx = tb.x - ox, sy = tb.y - oy, sz = tb.z - oz;
-          S v = S(0);
-          if (tb.w && sx >= 0 && sy >= 0 && sz >= 0 && sx < nbox && sy < nbox && sz < nbox) {
-            const int64_t snode = lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), level);
-            v = msl[a] * M[(snode * n + a) * K + k0 + col % k_per_tile];
-          }
-          ms_tile[w] = v;
-        }
-        __syncthreads();
-        if (beta < n) {
-          const S* Tb = T + int64_t(cls) * n * n + beta;
-          for (int a = 0; a < n; ++a) {
-            const S t = Tb[int64_t(a) * n];
-            const S* mrow = ms_tile + a * kCols;
-#pragma unroll
-            for (int c = 0; c < kCols; ++c) acc[c] += t * mrow[c];
-          }
-        }
+  for (int oi = 0; oi < 189; ++oi) {
+    const int4 o = offs[oi];
+    __syncthreads();
+    for (int w = threadIdx.x; w < n * kCols; w += blockDim.x) {
+      const int a = w / kCols, col = w % kCols;
+      const int4 tb = tbox[col];
+      const int sx = tb.x - o.x, sy = tb.y - o.y, sz = tb.z - o.z;
+      S v = S(0);
+      if (tb.w && sx >= 0 && sy >= 0 && sz >= 0 && sx < nbox && sy < nbox && sz < nbox) {
+        const int64_t snode = lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), level);
+        v = msl[a] * M[(snode * n + a) * K + k0 + col % kpt];
       }
+      ms_tile[w] = v;
+    }
+    __syncthreads();
+    if (beta < n) {
+      const S* Tb = T + int64_t(o.w) * n * n + beta;
+#pragma unroll 4
+      for (int a = 0; a < n; ++a) {
+        const S t = Tb[int64_t(a) * n];
+        const S* mrow = ms_tile + a * kCols;
+#pragma unroll
+        for (int c = 0; c < kCols; ++c) acc[c] += t * mrow[c];
+      }
+    }
+  }
   if (beta < n) {
     const S lsb = static_cast<const S*>(d.ls)[int64_t(level) * n + beta];
     S* Lc = static_cast<S*>(d.Lc);
@@ -90,7 +105,7 @@ __global__ void m2l_valu_kernel(const DevicePlan d, int level, int K, int boxes_
       const int4 tb = tbox[c];
       if (!tb.w) continue;
       const int64_t tnode = lvl0 + morton3(uint32_t(tb.x), uint32_t(tb.y), uint32_t(tb.z), level);
-      Lc[(tnode * n + beta) * K + k0 + c % k_per_tile] = lsb * acc[c];
+      Lc[(tnode * n + beta) * K + k0 + c % kpt] = lsb * acc[c];
     }
   }
 }
@@ -98,18 +113,27 @@ __global__ void m2l_valu_kernel(const DevicePlan d, int level, int K, int boxes_
 }  // namespace
 
 template <class S>
-void launch_m2l_valu(const DevicePlan& d, int level, int K, hipStream_t s) {
-  const int k_per_tile = K >= kCols ? kCols : K;
-  const int boxes_per_tile = kCols / k_per_tile;
-  const int64_t parents = int64_t(1) << (3 * (level - 1));
-  const dim3 grid(unsigned((parents + boxes_per_tile - 1) / boxes_per_tile), 8,
-                  unsigned((K + k_per_tile - 1) / k_per_tile));
+void launch_m2l_valu(const DevicePlan& d, int K, hipStream_t s) {
+  if (d.L < 2) return;
+  ValuLevels lv{};
+  lv.count = d.L - 1;
+  lv.k_per_tile = K >= kCols ? kCols : K;
+  lv.boxes_per_tile = kCols / lv.k_per_tile;
+  lv.kchunks = (K + lv.k_per_tile - 1) / lv.k_per_tile;
+  int begin = 0;
+  for (int i = 0; i < lv.count; ++i) {
+    const int64_t parents = int64_t(1) << (3 * (i + 1));
+    lv.tiles[i] = int((parents + lv.boxes_per_tile - 1) / lv.boxes_per_tile);
+    lv.block_begin[i] = begin;
+    begin += 8 * lv.tiles[i] * lv.kchunks;
+  }
+  lv.block_begin[lv.count] = begin;
   const int threads = ((d.n + 63) / 64) * 64;
   const size_t lds = size_t(d.n) * kCols * sizeof(S);
-  m2l_valu_kernel<S><<<grid, threads, lds, s>>>(d, level, K, boxes_per_tile, k_per_tile);
+  m2l_valu_kernel<S><<<unsigned(begin), threads, lds, s>>>(d, lv, K);
   HIPFMM_CHECK(hipGetLastError());
 }
-template void launch_m2l_valu<float>(const DevicePlan&, int, int, hipStream_t);
-template void launch_m2l_valu<double>(const DevicePlan&, int, int, hipStream_t);
+template void launch_m2l_valu<float>(const DevicePlan&, int, hipStream_t);
+template void launch_m2l_valu<double>(const DevicePlan&, int, hipStream_t);
 
 }  // namespace hipfmm

@@ -41,8 +41,8 @@ enum Ev {
   kEvD2H0,
   kEvD2H1,
   kEvUser,
-  kEvLeafM2L0,
-  kEvLeafM2L1,
+  kEvM2LKernel0,
+  kEvM2LKernel1,
   kEvCount
 };
 
@@ -298,6 +298,22 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   for (size_t i = 0; i < size_t(desc.m2l_class_count) * nn; ++i) T[i] = S(desc.m2l_matrices[i]);
   d.m2l_T = upload(P, T.data(), T.size());
   d.m2l_slot = upload(P, slot.data(), slot.size());
+  // The 189 list-2 offsets of each parity class (target coordinate parity
+  // q per axis: offsets in [q-3, q+2] minus the near cube).
+  std::vector<int4> parity_offsets(8 * 189, int4{0, 0, 0, 0});
+  if (L >= 2) {
+    for (int parity = 0; parity < 8; ++parity) {
+      const int q[3] = {parity & 1, (parity >> 1) & 1, (parity >> 2) & 1};
+      int k = 0;
+      for (int ox = q[0] - 3; ox <= q[0] + 2; ++ox)
+        for (int oy = q[1] - 3; oy <= q[1] + 2; ++oy)
+          for (int oz = q[2] - 3; oz <= q[2] + 2; ++oz) {
+            if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) <= 1) continue;
+            parity_offsets[parity * 189 + k++] = int4{ox, oy, oz, slot[hipfmm::offset_slot(ox, oy, oz)]};
+          }
+    }
+  }
+  d.parity_offsets = upload(P, parity_offsets.data(), parity_offsets.size());
   std::vector<S> ms(size_t(L + 1) * n), lsc(size_t(L + 1) * n);
   for (size_t i = 0; i < ms.size(); ++i) {
     ms[i] = S(desc.m2l_multipole_scaling[i]);
@@ -434,7 +450,7 @@ void finalize_timings(hipfmm_plan* P) {
   t.kernel_seconds = 1e-3 * elapsed(e[kEvH2D1], e[kEvL2P]);
   t.d2h_seconds = P->last_host ? 1e-3 * elapsed(e[kEvD2H0], e[kEvD2H1]) : 0.0;
   t.total_seconds = 1e-3 * elapsed(e[kEvH2D0], P->last_host ? e[kEvD2H1] : e[kEvL2P]);
-  t.m2l_leaf_seconds = P->d.L >= 2 ? 1e-3 * elapsed(e[kEvLeafM2L0], e[kEvLeafM2L1]) : 0.0;
+  t.m2l_kernel_seconds = P->d.L >= 2 ? 1e-3 * elapsed(e[kEvM2LKernel0], e[kEvM2LKernel1]) : 0.0;
   P->timings_pending = false;
 }
 
@@ -519,15 +535,13 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
   hipfmm::launch_zero<S>(static_cast<S*>(d.Lc), d.L >= 2 ? top : size_t(d.nodes) * d.n * K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvM2M], s));
   const bool mfma = sizeof(S) == 4 && P->m2l_mode != HIPFMM_M2L_VALU && P->mfma_ready;
-  for (int level = 2; level <= d.L; ++level) {
-    if (mfma) hipfmm::launch_m2l_split(d, level, K, P->m2l_mode, s);
-    if (level == d.L) HIPFMM_CHECK(hipEventRecord(e[kEvLeafM2L0], s));
-    if (mfma)
-      hipfmm::launch_m2l_mfma(d, level, K, P->m2l_mode, s);
-    else
-      hipfmm::launch_m2l_valu<S>(d, level, K, s);
-    if (level == d.L) HIPFMM_CHECK(hipEventRecord(e[kEvLeafM2L1], s));
-  }
+  if (mfma) hipfmm::launch_m2l_split(d, K, P->m2l_mode, s);
+  HIPFMM_CHECK(hipEventRecord(e[kEvM2LKernel0], s));
+  if (mfma)
+    hipfmm::launch_m2l_mfma(d, K, P->m2l_mode, s);
+  else
+    hipfmm::launch_m2l_valu<S>(d, K, s);
+  HIPFMM_CHECK(hipEventRecord(e[kEvM2LKernel1], s));
   HIPFMM_CHECK(hipEventRecord(e[kEvM2L], s));
   for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(d, level, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvL2L], s));

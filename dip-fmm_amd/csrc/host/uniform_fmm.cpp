@@ -429,7 +429,7 @@ void UniformFmm::record_timings() {
   l.kernel = t.kernel_seconds;
   l.d2h = t.d2h_seconds;
   l.total = t.total_seconds;
-  l.m2l_leaf = t.m2l_leaf_seconds;
+  l.m2l_kernel = t.m2l_kernel_seconds;
   l.evaluations = 1;
   auto& a = aggregate_timings_;
   a.h2d += t.h2d_seconds;
@@ -442,7 +442,7 @@ void UniformFmm::record_timings() {
   a.kernel += t.kernel_seconds;
   a.d2h += t.d2h_seconds;
   a.total += t.total_seconds;
-  a.m2l_leaf += t.m2l_leaf_seconds;
+  a.m2l_kernel += t.m2l_kernel_seconds;
   a.evaluations += 1;
 }
 

@@ -117,7 +117,7 @@ class UniformTree {
 struct EvaluationTimings {
   double h2d{0.0}, p2m{0.0}, m2m{0.0}, m2l{0.0}, l2l{0.0}, l2p{0.0}, p2p{0.0}, kernel{0.0}, d2h{0.0};
   double total{0.0};
-  double m2l_leaf{0.0};  // leaf-level M2L product kernel
+  double m2l_kernel{0.0};  // M2L product kernel (one launch, all levels)
   std::uint64_t evaluations{0};
 };
 

@@ -74,7 +74,7 @@ py::dict timings_dict(const EvaluationTimings& t) {
   d["kernel"] = t.kernel;
   d["d2h"] = t.d2h;
   d["total"] = t.total;
-  d["m2l_leaf"] = t.m2l_leaf;
+  d["m2l_kernel"] = t.m2l_kernel;
   d["evaluations"] = t.evaluations;
   return d;
 }

@@ -141,7 +141,7 @@ typedef struct hipfmm_timings {
   double kernel_seconds;/* first device stage to last */
   double d2h_seconds;
   double total_seconds;
-  double m2l_leaf_seconds; /* the leaf-level M2L product kernel alone */
+  double m2l_kernel_seconds; /* the M2L product kernel alone (one launch, all levels) */
 } hipfmm_timings;
 
 /* Message of the last failure on this thread ("" if none). */
