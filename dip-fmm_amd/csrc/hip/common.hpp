@@ -49,19 +49,35 @@ __host__ __device__ inline int coefficient_count(int p) { return (p + 1) * (p + 
 __host__ __device__ inline int64_t level_offset(int level) { return ((int64_t(1) << (3 * level)) - 1) / 7; }
 __host__ __device__ inline int offset_slot(int dx, int dy, int dz) { return (dx + 3) * 49 + (dy + 3) * 7 + (dz + 3); }
 
+// Spread the low 10 bits of v to every third bit (bit b -> bit 3b).
+__host__ __device__ inline uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+// Morton index of box (x, y, z) at a level of `bits` bits per axis (<= 10):
+// bit b of x, y, z goes to bit 3b, 3b+1, 3b+2.
 __host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z, int bits) {
-  uint32_t m = 0;
-  for (int b = 0; b < bits; ++b)
-    m |= (((x >> b) & 1u) << (3 * b)) | (((y >> b) & 1u) << (3 * b + 1)) | (((z >> b) & 1u) << (3 * b + 2));
-  return m;
+  const uint32_t mask = bits >= 10 ? 0x3ffu : (1u << bits) - 1u;
+  return spread3(x & mask) | (spread3(y & mask) << 1) | (spread3(z & mask) << 2);
+}
+// Inverse of spread3: gather every third bit (bit 3b -> bit b).
+__host__ __device__ inline uint32_t compact3(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v ^ (v >> 2)) & 0x030c30c3u;
+  v = (v ^ (v >> 4)) & 0x0300f00fu;
+  v = (v ^ (v >> 8)) & 0x030000ffu;
+  v = (v ^ (v >> 16)) & 0x3ffu;
+  return v;
 }
 __host__ __device__ inline void demorton3(uint32_t m, int bits, int& x, int& y, int& z) {
-  x = y = z = 0;
-  for (int b = 0; b < bits; ++b) {
-    x |= int((m >> (3 * b)) & 1u) << b;
-    y |= int((m >> (3 * b + 1)) & 1u) << b;
-    z |= int((m >> (3 * b + 2)) & 1u) << b;
-  }
+  const uint32_t mask = bits >= 10 ? 0x3ffu : (1u << bits) - 1u;
+  x = int(compact3(m) & mask);
+  y = int(compact3(m >> 1) & mask);
+  z = int(compact3(m >> 2) & mask);
 }
 
 template <class S>
@@ -125,6 +141,7 @@ struct DevicePlan {
   void* T_lo = nullptr;
   void* ls_pad = nullptr; // (L+1) x bpad float local scaling (0 beyond n)
   float* lvl_scale = nullptr; // per level operand scale for FP16 mode
+  const void* zero_block = nullptr;  // 1 KiB of zeros: the operand of sources outside the grid
   int apad = 0, bpad = 0;
 };
 

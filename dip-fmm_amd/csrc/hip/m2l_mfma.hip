@@ -7,20 +7,25 @@
 // gathered source columns (depth a, columns = target box x moment state).
 //
 // Work decomposition (one launch covers every level 2..L):
-//   * a workgroup holds 4 waves and owns 4*NB tiles of 32 output columns of
+//   * a workgroup holds W waves and owns W*NB tiles of 32 output columns of
 //     one parity class at one level; each wave owns NB tiles and every row
 //     tile of L for them (n rounded up to 32 rows), so each local is written
 //     once and no atomics are needed;
-//   * the workgroup walks the 189 list-2 offsets of its parity; the class
-//     matrix of each offset is streamed through LDS in chunks of CH depth
-//     steps, double buffered, and read by all 4 waves (A is the operand with
-//     the most reuse: every box of the parity shares it);
-//   * B fragments (source multipoles) are prefetched one step ahead from
-//     HBM/L2; a wave skips the products of an offset whose source boxes all
-//     lie outside the level's grid;
+//   * the workgroup walks (depth step ks, offset o) with ks outermost: for a
+//     fixed ks the sources of all resident workgroups are one 16-deep slice
+//     of the multipoles, which stays in L2 / the last-level cache while the
+//     189 offsets of the parity are applied;
+//   * offsets are ordered by their coordinate parity (the host table), so
+//     consecutive offsets of the same class read sources shifted by one
+//     lattice step of the target parity and re-hit L2;
+//   * operands of a step (the class matrix slice T_o[ks], hi and lo, shared
+//     by the workgroup; each wave's source fragments) are fetched straight
+//     into LDS by global_load_lds DMA, kStages - 1 steps ahead, into a ring
+//     of kStages slots held in separate LDS arrays; one barrier per step
+//     publishes a slot to every wave and frees the oldest one;
 //   * block b -> parity b % 8, so the 8 XCDs each serve one parity class and
-//     keep its 189 class matrices in their own L2; consecutive workgroups of
-//     one XCD hold spatially consecutive tiles.
+//     keep its class matrices in their own L2; consecutive workgroups of one
+//     XCD hold spatially consecutive tiles.
 //
 // Precision: FP32 operands are split x = hi + lo with hi = bf16(x) and
 // lo = bf16(x - hi) (16 significant bits), and each product is formed as
@@ -29,10 +34,10 @@
 // Data layout:
 //   T_hi/T_lo  [class][depth step][row tile][lane][8] bf16 -- MFMA A-fragment
 //              order (lane i: row i%32, depth 8*(i/32)..+7 of the 16-deep
-//              step), so a chunk of depth steps is one contiguous block
+//              step), so the slice of one step is MT contiguous 1 KiB blocks
 //   mt_hi/lo   [node][depth step][state][16] bf16 -- a B fragment (column =
 //              lane%32 -> state, depth 8*(i/32)..+7) is one 16-byte load and
-//              the 32 columns of a step form one contiguous 1 KB block
+//              the 32 columns of a step form one contiguous 1 KiB block
 #include "common.hpp"
 
 namespace hipfmm {
@@ -42,8 +47,31 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kNB = 2;  // 32-column tiles per wave
-constexpr int kCH = 1;  // depth steps per LDS chunk
+constexpr int kNB = 2;      // 32-column tiles per wave
+constexpr int kWaves = 4;   // waves per workgroup
+constexpr int kStages = 4;  // LDS ring slots (operands fetched kStages - 1 steps ahead)
+constexpr int kParityOffsets = 189;
+
+// s_waitcnt immediate (gfx9 encoding): at most `vm` vector-memory operations
+// outstanding, every LDS operation complete.
+constexpr int wait_vm_lgkm0(int vm) { return (vm & 15) | ((vm >> 4) << 14) | (7 << 4); }
+
+// 16 bytes per lane from global memory straight into LDS: lane i writes
+// lds_wave_base[i] (M0 holds the wave-uniform LDS address; one wait state
+// separates the M0 write from its use).  Issued as inline assembly so the
+// compiler's wait insertion does not track it: that tracking loses the ring
+// slot a DMA wrote at the step's branches and would then wait for every DMA
+// in flight before each LDS read.  Ordering is explicit instead: the
+// pre-barrier s_waitcnt of each step.  The compiler keeps no value of its
+// own in M0 in this kernel (it uses M0 only for its own LDS DMA and message
+// instructions, which the kernel does not contain).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void lds_dma16(const uint4* src, uint4* lds_wave_base) {
+  const uint32_t lds = uint32_t(uintptr_t((__attribute__((address_space(3))) uint4*)lds_wave_base));
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds) : "memory", "m0");
+}
+#pragma clang diagnostic pop
 
 __host__ __device__ inline uint16_t bf16_bits(float x) {
   uint32_t u = __builtin_bit_cast(uint32_t, x);
@@ -100,164 +128,183 @@ struct LevelTable {
   int kchunks[10];
 };
 
-template <int MT>
-__global__ void __launch_bounds__(256, 1) m2l_mfma_kernel(const DevicePlan d, const LevelTable lt, int K) {
-  constexpr int UNITS_PER_KS = MT * 64;  // uint4 per depth step per half (hi or lo)
-  __shared__ uint4 abuf[2][2][kCH][UNITS_PER_KS];  // [buffer][hi/lo][step][row tile x lane]
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// Per-lane view of the columns of the NB tiles one wave owns.
+template <int NB>
+struct Columns {
+  bool valid[NB];
+  int tx[NB], ty[NB], tz[NB], kk[NB];
+};
+
+template <int NB>
+__device__ __forceinline__ bool source_inside(const Columns<NB>& c, int nb, int4 o, int nbox, int& sx, int& sy,
+                                              int& sz) {
+  sx = c.tx[nb] - o.x;
+  sy = c.ty[nb] - o.y;
+  sz = c.tz[nb] - o.z;
+  return c.valid[nb] && sx >= 0 && sy >= 0 && sz >= 0 && sx < nbox && sy < nbox && sz < nbox;
+}
+
+struct StepGeometry {
+  int K, KS, level, nbox, nsteps;
+  int64_t lvl0;
+};
+
+// Fetch the operands of step `step` (clamped to the last step, so every
+// step issues the same number of DMA operations per wave) into one ring slot.
+template <int MT, int NB, int W>
+__device__ __forceinline__ void issue_step(int step, uint4* a_slot, uint4* b_slot, const int4* offs,
+                                           const DevicePlan& d, const Columns<NB>& c, const StepGeometry& sg,
+                                           int wave, int lane) {
+  constexpr int kAChunks = 2 * MT, kAPerWave = (kAChunks + W - 1) / W;
+  const int st = step < sg.nsteps ? step : sg.nsteps - 1;
+  const int ks = st / kParityOffsets, oi = st - kParityOffsets * ks;
+  const int4 o = offs[oi];
+  const int64_t tbase = (int64_t(o.w) * sg.KS + ks) * MT * 64 + lane;
+#pragma unroll
+  for (int i = 0; i < kAPerWave; ++i) {
+    int ch = wave + W * i;
+    if (ch >= kAChunks) ch -= kAChunks;  // repeats another wave's chunk: same bytes to the same place
+    const int half = ch >= MT ? 1 : 0, m = ch - half * MT;
+    const uint4* src = static_cast<const uint4*>(half ? d.T_lo : d.T_hi) + tbase + int64_t(m) * 64;
+    lds_dma16(src, a_slot + ch * 64);
+  }
+  const uint4* zero = static_cast<const uint4*>(d.zero_block) + lane;
+  const uint4* mhi = static_cast<const uint4*>(d.mt_hi);
+  const uint4* mlo = static_cast<const uint4*>(d.mt_lo);
+  const int g = lane >> 5;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    int sx, sy, sz;
+    const bool inside = source_inside(c, nb, o, sg.nbox, sx, sy, sz);
+    // Branch-free: the Morton index of an outside box is computed and discarded.
+    const int64_t snode = sg.lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), sg.level);
+    const int64_t at = ((snode * sg.KS + ks) * sg.K + c.kk[nb]) * 2 + g;
+    uint4* dst = b_slot + (wave * NB + nb) * 128;
+    lds_dma16(inside ? mhi + at : zero, dst);
+    lds_dma16(inside ? mlo + at : zero, dst + 64);
+  }
+}
+
+// The 3*MT*NB products of one step from a ring slot; a wave whose columns
+// all lie outside the grid for this offset skips them.
+template <int MT, int NB>
+__device__ __forceinline__ void compute_step(int step, const uint4* a_slot, const uint4* b_slot, const int4* offs,
+                                             const Columns<NB>& c, const StepGeometry& sg, int wave, int lane,
+                                             f32x16 (&acc)[NB][MT]) {
+  const int oi = step % kParityOffsets;
+  const int4 o = offs[oi];
+  bool any = false;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    int sx, sy, sz;
+    any |= source_inside(c, nb, o, sg.nbox, sx, sy, sz);
+  }
+  if (!__any(any)) return;
+  bf16x8 bh[NB], bl[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const uint4* src = b_slot + (wave * NB + nb) * 128 + lane;
+    bh[nb] = __builtin_bit_cast(bf16x8, src[0]);
+    bl[nb] = __builtin_bit_cast(bf16x8, src[64]);
+  }
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    const bf16x8 ah = __builtin_bit_cast(bf16x8, a_slot[m * 64 + lane]);
+    const bf16x8 al = __builtin_bit_cast(bf16x8, a_slot[(MT + m) * 64 + lane]);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[nb], acc[nb][m], 0, 0, 0);
+      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[nb], acc[nb][m], 0, 0, 0);
+      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[nb], acc[nb][m], 0, 0, 0);
+    }
+  }
+}
+
+template <int MT, int NB, int W>
+__global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d, const LevelTable lt, int K) {
+  static_assert(kStages == 4, "the step loop below is unrolled for four ring slots");
+  constexpr int kAUnits = 2 * MT * 64, kBUnits = W * NB * 2 * 64;
+  constexpr int kAPerWave = (2 * MT + W - 1) / W;
+  constexpr int kDmaPerStep = kAPerWave + 2 * NB;
+  static_assert((kStages - 2) * kDmaPerStep <= 63, "vmcnt holds at most 63 outstanding operations");
+  // Ring slots: the class-matrix slice, then the waves' source fragments.
+  __shared__ uint4 r0[kAUnits + kBUnits], r1[kAUnits + kBUnits], r2[kAUnits + kBUnits], r3[kAUnits + kBUnits];
+  __shared__ int4 offs[kParityOffsets];
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
   int li = lt.count - 1;
   while (li > 0 && int(blockIdx.x) < lt.block_begin[li]) --li;
   const int level = lt.first_level + li;
   const int lb = int(blockIdx.x) - lt.block_begin[li];
   const int parity = lb & 7, wgp = lb >> 3;
   const int px = parity & 1, py = (parity >> 1) & 1, pz = (parity >> 2) & 1;
-  const int n = d.n, AP = d.apad, BP = d.bpad, KS = AP / 16;
-  const int NCH = (KS + kCH - 1) / kCH;
-  const int nbox = 1 << level;
+  const int n = d.n, BP = d.bpad;
+  StepGeometry sg;
+  sg.K = K;
+  sg.KS = d.apad / 16;
+  sg.level = level;
+  sg.nbox = 1 << level;
+  sg.nsteps = sg.KS * kParityOffsets;
+  sg.lvl0 = level_offset(level);
   const int64_t parents = int64_t(1) << (3 * (level - 1));
   const int cpb = lt.cols_per_box[li], kchunks = lt.kchunks[li], tiles = lt.tiles[li];
   const int boxes_per_tile = 32 / cpb;
   const int col = lane & 31, g = lane >> 5;
-  // Per owned tile: column -> (target box, state).
-  bool valid[kNB];
-  int tx[kNB], ty[kNB], tz[kNB], kk[kNB];
+  Columns<NB> c;
 #pragma unroll
-  for (int nb = 0; nb < kNB; ++nb) {
-    const int tile = (wgp * 4 + wave) * kNB + nb;
+  for (int nb = 0; nb < NB; ++nb) {
+    const int tile = (wgp * W + wave) * NB + nb;
     const int bg = tile / kchunks, kc = tile % kchunks;
     const int64_t q = int64_t(bg) * boxes_per_tile + col / cpb;
-    kk[nb] = kc * cpb + col % cpb;
-    valid[nb] = tile < tiles && q < parents && kk[nb] < K;
+    c.kk[nb] = kc * cpb + col % cpb;
+    c.valid[nb] = tile < tiles && q < parents && c.kk[nb] < K;
     int x = 0, y = 0, z = 0;
-    if (valid[nb]) demorton3(uint32_t(q), level - 1, x, y, z);
-    tx[nb] = 2 * x + px;
-    ty[nb] = 2 * y + py;
-    tz[nb] = 2 * z + pz;
+    if (c.valid[nb]) demorton3(uint32_t(q), level - 1, x, y, z);
+    c.tx[nb] = 2 * x + px;
+    c.ty[nb] = 2 * y + py;
+    c.tz[nb] = 2 * z + pz;
   }
-  const int64_t lvl0 = level_offset(level);
-  const int4* offs = d.parity_offsets + parity * 189;
-  const uint4* Thi = static_cast<const uint4*>(d.T_hi);
-  const uint4* Tlo = static_cast<const uint4*>(d.T_lo);
-  const uint4* mhi = static_cast<const uint4*>(d.mt_hi);
-  const uint4* mlo = static_cast<const uint4*>(d.mt_lo);
-  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  for (int i = threadIdx.x; i < kParityOffsets; i += W * 64) offs[i] = d.parity_offsets[parity * kParityOffsets + i];
+  __syncthreads();
 
-  // A staging: step s = (offset oi, chunk c); this thread's share of the
-  // chunk's A fragments (hi then lo halves) is loaded into registers one step
-  // ahead and stored into the free LDS buffer after the current step's
-  // products.
-  constexpr int STAGE_UNITS = 2 * kCH * UNITS_PER_KS;
-  constexpr int PER_THREAD = (STAGE_UNITS + 255) / 256;
-  uint4 stage[PER_THREAD];
-#define HIPFMM_STAGE_LOAD(step)                                                                     \
-  do {                                                                                               \
-    const int oi_ = (step) / NCH, c_ = (step) % NCH;                                                 \
-    const int cls_ = offs[oi_].w;                                                                    \
-    const int ks0_ = c_ * kCH, units_ = min(kCH, KS - ks0_) * UNITS_PER_KS;                         \
-    const int64_t base_ = (int64_t(cls_) * KS + ks0_) * UNITS_PER_KS;                                \
-    _Pragma("unroll") for (int i = 0; i < PER_THREAD; ++i) {                                        \
-      const int u = int(threadIdx.x) + 256 * i;                                                      \
-      const int half = u / (kCH * UNITS_PER_KS), r = u % (kCH * UNITS_PER_KS);                       \
-      stage[i] = zero4;                                                                              \
-      if (u < STAGE_UNITS && r < units_) stage[i] = (half ? Tlo : Thi)[base_ + r];                   \
-    }                                                                                                \
-  } while (0)
-#define HIPFMM_STAGE_STORE(buf)                                                                     \
-  do {                                                                                               \
-    _Pragma("unroll") for (int i = 0; i < PER_THREAD; ++i) {                                        \
-      const int u = int(threadIdx.x) + 256 * i;                                                      \
-      if (u < STAGE_UNITS) (&abuf[(buf)][0][0][0])[u] = stage[i];                                    \
-    }                                                                                                \
-  } while (0)
-  // B fragments of one step for this lane's NB columns.
-  uint4 bcur[kNB][kCH][2], bnext[kNB][kCH][2];
-  auto load_b_next = [&](int s) -> bool {
-    const int oi = s / NCH, c = s % NCH;
-    const int4 o = offs[oi];
-    const int ks0 = c * kCH, nks = min(kCH, KS - ks0);
-    bool any = false;
+  f32x16 acc[NB][MT];
 #pragma unroll
-    for (int nb = 0; nb < kNB; ++nb) {
-      const int sx = tx[nb] - o.x, sy = ty[nb] - o.y, sz = tz[nb] - o.z;
-      const bool sv = valid[nb] && sx >= 0 && sy >= 0 && sz >= 0 && sx < nbox && sy < nbox && sz < nbox;
-      any |= sv;
-      const int64_t snode = sv ? lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), level) : 0;
-#pragma unroll
-      for (int j = 0; j < kCH; ++j) {
-        const int64_t at = (((snode * KS + ks0 + j) * K + kk[nb]) * 2 + g);
-        const bool ok = sv && j < nks;
-        bnext[nb][j][0] = ok ? mhi[at] : zero4;
-        bnext[nb][j][1] = ok ? mlo[at] : zero4;
-      }
-    }
-    return __any(any) != 0;
-  };
-
-  f32x16 acc[kNB][MT];
-#pragma unroll
-  for (int nb = 0; nb < kNB; ++nb)
+  for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[nb][m][r] = 0.0f;
 
-  const int nsteps = 189 * NCH;
-  HIPFMM_STAGE_LOAD(0);
-  HIPFMM_STAGE_STORE(0);
-  bool active = load_b_next(0);
-#pragma unroll
-  for (int nb = 0; nb < kNB; ++nb)
-#pragma unroll
-    for (int j = 0; j < kCH; ++j) {
-      bcur[nb][j][0] = bnext[nb][j][0];
-      bcur[nb][j][1] = bnext[nb][j][1];
-    }
-  __syncthreads();
-  for (int s = 0; s < nsteps; ++s) {
-    const int buf = s & 1;
-    const bool more = s + 1 < nsteps;
-    if (more) HIPFMM_STAGE_LOAD(s + 1);
-    const bool active_next = more ? load_b_next(s + 1) : false;
-    if (active) {
-      const int nks = min(kCH, KS - (s % NCH) * kCH);
-#pragma unroll
-      for (int j = 0; j < kCH; ++j) {
-        if (j >= nks) break;
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-          const bf16x8 ah = as_bf16x8(abuf[buf][0][j][m * 64 + lane]);
-          const bf16x8 al = as_bf16x8(abuf[buf][1][j][m * 64 + lane]);
-#pragma unroll
-          for (int nb = 0; nb < kNB; ++nb) {
-            const bf16x8 bh = as_bf16x8(bcur[nb][j][0]);
-            const bf16x8 bl = as_bf16x8(bcur[nb][j][1]);
-            acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[nb][m], 0, 0, 0);
-            acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[nb][m], 0, 0, 0);
-            acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[nb][m], 0, 0, 0);
-          }
-        }
-      }
-    }
-    if (more) HIPFMM_STAGE_STORE(buf ^ 1);
-    __syncthreads();
-#pragma unroll
-    for (int nb = 0; nb < kNB; ++nb)
-#pragma unroll
-      for (int j = 0; j < kCH; ++j) {
-        bcur[nb][j][0] = bnext[nb][j][0];
-        bcur[nb][j][1] = bnext[nb][j][1];
-      }
-    active = active_next;
+  issue_step<MT, NB, W>(0, r0, r0 + kAUnits, offs, d, c, sg, wave, lane);
+  issue_step<MT, NB, W>(1, r1, r1 + kAUnits, offs, d, c, sg, wave, lane);
+  issue_step<MT, NB, W>(2, r2, r2 + kAUnits, offs, d, c, sg, wave, lane);
+  // Step s: wait for this wave's DMA of step s (the two younger steps may
+  // stay in flight) and for its LDS reads of step s-1, barrier (every wave's
+  // share of slot s is in LDS; slot s-1 is free), refill slot s-1 with step
+  // s+3, then multiply out of slot s.  The inline-asm barrier keeps the
+  // compiler from moving LDS accesses across it.
+#define HIPFMM_RING_STEP(S_, CUR, NXT)                                                  \
+  if ((S_) < sg.nsteps) {                                                               \
+    __builtin_amdgcn_s_waitcnt(wait_vm_lgkm0((kStages - 2) * kDmaPerStep));             \
+    asm volatile("s_barrier" ::: "memory");                                             \
+    issue_step<MT, NB, W>((S_) + kStages - 1, r##NXT, r##NXT + kAUnits, offs, d, c, sg, wave, lane); \
+    compute_step<MT, NB>((S_), r##CUR, r##CUR + kAUnits, offs, c, sg, wave, lane, acc);           \
   }
-#undef HIPFMM_STAGE_LOAD
-#undef HIPFMM_STAGE_STORE
+  for (int s = 0; s < sg.nsteps; s += kStages) {
+    HIPFMM_RING_STEP(s, 0, 3)
+    HIPFMM_RING_STEP(s + 1, 1, 0)
+    HIPFMM_RING_STEP(s + 2, 2, 1)
+    HIPFMM_RING_STEP(s + 3, 3, 2)
+  }
+#undef HIPFMM_RING_STEP
+  __builtin_amdgcn_s_waitcnt(0);  // the trailing (clamped) DMAs land before the workgroup's LDS is released
+
   const float* lsp = static_cast<const float*>(d.ls_pad) + int64_t(level) * BP;
   float* Lc = static_cast<float*>(d.Lc);
 #pragma unroll
-  for (int nb = 0; nb < kNB; ++nb) {
-    if (!valid[nb]) continue;
-    const int64_t tnode = lvl0 + morton3(uint32_t(tx[nb]), uint32_t(ty[nb]), uint32_t(tz[nb]), level);
-    float* out = Lc + tnode * int64_t(n) * K + kk[nb];
+  for (int nb = 0; nb < NB; ++nb) {
+    if (!c.valid[nb]) continue;
+    const int64_t tnode = sg.lvl0 + morton3(uint32_t(c.tx[nb]), uint32_t(c.ty[nb]), uint32_t(c.tz[nb]), level);
+    float* out = Lc + tnode * int64_t(n) * K + c.kk[nb];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -281,7 +328,7 @@ LevelTable level_table(const DevicePlan& d, int K) {
     const int64_t parents = int64_t(1) << (3 * (level - 1));
     const int boxes_per_tile = 32 / cpb;
     const int tiles = int((parents + boxes_per_tile - 1) / boxes_per_tile) * kchunks;
-    const int bpp = (tiles + 4 * kNB - 1) / (4 * kNB);
+    const int bpp = (tiles + kWaves * kNB - 1) / (kWaves * kNB);
     lt.block_begin[i] = begin;
     lt.blocks_per_parity[i] = bpp;
     lt.tiles[i] = tiles;
@@ -296,7 +343,7 @@ LevelTable level_table(const DevicePlan& d, int K) {
 template <int MT>
 void launch_mfma_t(const DevicePlan& d, int K, hipStream_t s) {
   const LevelTable lt = level_table(d, K);
-  m2l_mfma_kernel<MT><<<unsigned(lt.block_begin[lt.count]), 256, 0, s>>>(d, lt, K);
+  m2l_mfma_kernel<MT, kNB, kWaves><<<unsigned(lt.block_begin[lt.count]), kWaves * 64, 0, s>>>(d, lt, K);
   HIPFMM_CHECK(hipGetLastError());
 }
 
@@ -363,6 +410,8 @@ void prepare_mfma_operators(DevicePlan& d, const std::vector<double>& matrices, 
     bytes += count;
     return ptr;
   };
+  const std::vector<uint16_t> zeros(512, 0);
+  d.zero_block = up(zeros.data(), zeros.size() * 2);
   d.T_hi = up(hi.data(), hi.size() * 2);
   d.T_lo = up(lo.data(), lo.size() * 2);
   d.ls_pad = up(lsp.data(), lsp.size() * 4);

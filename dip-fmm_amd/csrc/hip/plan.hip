@@ -299,18 +299,23 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   d.m2l_T = upload(P, T.data(), T.size());
   d.m2l_slot = upload(P, slot.data(), slot.size());
   // The 189 list-2 offsets of each parity class (target coordinate parity
-  // q per axis: offsets in [q-3, q+2] minus the near cube).
+  // q per axis: offsets in [q-3, q+2] minus the near cube), ordered by the
+  // offset's own coordinate parity: within one such group the sources of
+  // same-parity targets are shared between consecutive offsets.
   std::vector<int4> parity_offsets(8 * 189, int4{0, 0, 0, 0});
   if (L >= 2) {
     for (int parity = 0; parity < 8; ++parity) {
       const int q[3] = {parity & 1, (parity >> 1) & 1, (parity >> 2) & 1};
       int k = 0;
-      for (int ox = q[0] - 3; ox <= q[0] + 2; ++ox)
-        for (int oy = q[1] - 3; oy <= q[1] + 2; ++oy)
-          for (int oz = q[2] - 3; oz <= q[2] + 2; ++oz) {
-            if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) <= 1) continue;
-            parity_offsets[parity * 189 + k++] = int4{ox, oy, oz, slot[hipfmm::offset_slot(ox, oy, oz)]};
-          }
+      for (int group = 0; group < 8; ++group)
+        for (int ox = q[0] - 3; ox <= q[0] + 2; ++ox)
+          for (int oy = q[1] - 3; oy <= q[1] + 2; ++oy)
+            for (int oz = q[2] - 3; oz <= q[2] + 2; ++oz) {
+              if (std::max({std::abs(ox), std::abs(oy), std::abs(oz)}) <= 1) continue;
+              if (((ox & 1) | ((oy & 1) << 1) | ((oz & 1) << 2)) != group) continue;
+              parity_offsets[parity * 189 + k++] = int4{ox, oy, oz, slot[hipfmm::offset_slot(ox, oy, oz)]};
+            }
+      if (k != 189) throw std::logic_error("list-2 offset enumeration");
     }
   }
   d.parity_offsets = upload(P, parity_offsets.data(), parity_offsets.size());
