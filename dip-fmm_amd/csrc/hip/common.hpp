@@ -126,6 +126,10 @@ struct DevicePlan {
   void* ms = nullptr;        // (L+1) x n
   void* ls = nullptr;
   ShiftCsr m2m, l2l;
+  // The same operators dense and transposed, [(level*8 + class)][col][NP]
+  // with NP = n rounded up to 8 (rows n..NP-1 zero), levels 0..L.
+  void* m2m_dense = nullptr;
+  void* l2l_dense = nullptr;
   // state (capacity K)
   int kcap = 0;
   void* moments_in = nullptr;  // host-path staging: K x ns x 3

@@ -1,10 +1,16 @@
 // Far-field kernels of the HipFull path: moment gather, leaf P2M, level-wise
 // M2M and L2L, and leaf L2P fused with the near-field combine and the unsort
-// to user target order.  All are HBM-bandwidth kernels: one workgroup owns
-// one box, reads its inputs once and writes its outputs once, with no atomics.
+// to user target order.  One workgroup owns one box (P2M, L2P) or one parent
+// box (M2M, L2L), reads its inputs once and writes its outputs once, with no
+// atomics.
 //
-// Reference operations restated here (values computed on the fly instead of
-// streamed from stored entry lists):
+// Each operation is a small dense product per box, computed as a register-
+// tiled GEMM on the vector ALUs (FP32 or FP64 FMA, no reduced precision):
+// a thread owns an 8-row x 4-state tile of the output and, per depth step,
+// reads 8 operator entries and 4 state values as 16-byte vectors from LDS or
+// L2 (3 vector loads per 32 FMAs at FP32).
+//
+// Reference operations restated here:
 //   permute_moments / prepare_moments_float   src/cuda_fmm.cu:56-64, src/far_field.cpp:334-365
 //   P2M   M_a = (-1)^|a| sum_j sum_{c: a_c>0} m_jc d_j^(a-e_c)/(a-e_c)!
 //         src/static_operators.cpp:186-220, src/operators.cpp:16-52
@@ -13,6 +19,8 @@
 //   L2P   H_c = -sum_{b: b_c>0} L_b dx^(b-e_c)/(b-e_c)!
 //         src/static_operators.cpp:300-335
 //   combine + unsort  src/cuda_fmm.cu:503-516, src/uniform_fmm.cpp:1439-1452
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace hipfmm {
@@ -20,7 +28,11 @@ namespace hipfmm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kOutputsPerThread = 8;  // outputs per thread in the coefficient kernels
+constexpr int kRA = 8;  // output rows per thread tile
+constexpr int kRK = 4;  // moment states per thread tile
+
+__host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+__host__ __device__ inline size_t align16(size_t bytes) { return (bytes + 15) & ~size_t(15); }
 
 // d^g/g! for one axis, g <= p, by the recurrence x^i/i! = x^(i-1)/(i-1)! * x / i.
 template <class S>
@@ -28,6 +40,68 @@ __device__ inline S axis_power(S x, int g) {
   S v = S(1);
   for (int i = 1; i <= g; ++i) v = v * x / S(i);
   return v;
+}
+
+// R consecutive scalars from 16-byte aligned memory (LDS or global).
+template <class S, int R>
+__device__ __forceinline__ void load_vec(const S* p, S (&out)[R]) {
+  if constexpr (sizeof(S) == 4) {
+#pragma unroll
+    for (int i = 0; i < R; i += 4) {
+      const float4 t = *reinterpret_cast<const float4*>(p + i);
+      out[i] = t.x;
+      out[i + 1] = t.y;
+      out[i + 2] = t.z;
+      out[i + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; i += 2) {
+      const double2 t = *reinterpret_cast<const double2*>(p + i);
+      out[i] = t.x;
+      out[i + 1] = t.y;
+    }
+  }
+}
+
+// acc[i][j] += sum_d A[d*lda + i] * B[d*ldb + j]: A and B already point at
+// the thread's first row / state; both 16-byte aligned per depth step.
+template <class S>
+__device__ __forceinline__ void tile_gemm(const S* __restrict__ A, int lda, const S* __restrict__ B, int ldb,
+                                          int depth, S (&acc)[kRA][kRK]) {
+#pragma unroll 4
+  for (int dd = 0; dd < depth; ++dd) {
+    S a[kRA], b[kRK];
+    load_vec<S, kRA>(A + size_t(dd) * lda, a);
+    load_vec<S, kRK>(B + size_t(dd) * ldb, b);
+#pragma unroll
+    for (int i = 0; i < kRA; ++i)
+#pragma unroll
+      for (int j = 0; j < kRK; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+  }
+}
+
+// Rows [n_rows) x columns [KCn) of a box's coefficient block C[row][ld]
+// (C offset to the first column) from a tile accumulator; kAdd accumulates
+// into the existing values.
+template <class S, bool kAdd>
+__device__ __forceinline__ void store_tile(S* __restrict__ C, int row0, int n_rows, int k0, int KCn, int ld,
+                                           const S (&acc)[kRA][kRK]) {
+#pragma unroll
+  for (int i = 0; i < kRA; ++i) {
+    const int row = row0 + i;
+    if (row >= n_rows) break;
+    S* dst = C + size_t(row) * ld + k0;
+#pragma unroll
+    for (int j = 0; j < kRK; ++j) {
+      if (k0 + j >= KCn) break;
+      const S v = acc[i][j];
+      if constexpr (kAdd)
+        dst[j] += v;
+      else
+        dst[j] = v;
+    }
+  }
 }
 
 template <class S>
@@ -43,163 +117,325 @@ __global__ void __launch_bounds__(kThreads) gather_kernel(const S* __restrict__ 
   dst[2 * K] = src[2] * scale;
 }
 
-// One workgroup per (leaf, group of kThreads*kOutputsPerThread outputs).
-// Outputs o = alpha*K + k.  Sources are processed in chunks whose monomials
-// d^g/g! (|g| <= p-1) and moments are staged in LDS.
+// Shared LDS layout of the monomial-based kernels (P2M, L2P).
+struct MonoLayout {
+  int NP, KP, JC, TCP;
+  size_t off_tab, off_ex, off_px, off_mono, off_a, off_b, bytes;
+};
+
+// P2M layout: tab[n] int4, ex[nm] int4, px[JC][3][p], mono[JC][nm],
+// A[3*JC][NP] (row = (source, component), column = alpha), B[3*JC][KP].
 template <class S>
-__global__ void __launch_bounds__(kThreads) p2m_kernel(const DevicePlan d, int K, int chunk) {
-  extern __shared__ unsigned char smem_raw[];
-  S* mono = reinterpret_cast<S*>(smem_raw);   // chunk x nm
-  S* mom = mono + chunk * d.nm;                // chunk x 3 x K
+MonoLayout p2m_layout(const DevicePlan& d, int KC) {
+  MonoLayout l{};
+  l.NP = round_up(d.n, kRA);
+  l.KP = round_up(KC, kRK);
+  l.JC = sizeof(S) == 4 ? 16 : 8;
+  size_t o = 0;
+  l.off_tab = o;
+  o += align16(size_t(d.n) * sizeof(int4));
+  l.off_ex = o;
+  o += align16(size_t(std::max(d.nm, 1)) * sizeof(int4));
+  l.off_px = o;
+  o += align16(size_t(l.JC) * 3 * std::max(d.p, 1) * sizeof(S));
+  l.off_mono = o;
+  o += align16(size_t(l.JC) * std::max(d.nm, 1) * sizeof(S));
+  l.off_a = o;
+  o += align16(size_t(3 * l.JC) * l.NP * sizeof(S));
+  l.off_b = o;
+  o += align16(size_t(3 * l.JC) * l.KP * sizeof(S));
+  l.bytes = o;
+  return l;
+}
+
+// One workgroup per (leaf, chunk of KC states):
+//   M[alpha][k] = sign(alpha) sum_{(j,c)} A[(j,c)][alpha] B[(j,c)][k]
+// with A[(j,c)][alpha] = d_j^(alpha-e_c)/(alpha-e_c)! (0 unless alpha_c > 0)
+// and B[(j,c)][k] = m_jc (state k); sources in chunks of JC.
+template <class S>
+__global__ void __launch_bounds__(1024) p2m_kernel(const DevicePlan d, int K, int KC, MonoLayout l) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int4* tab = reinterpret_cast<int4*>(smem_raw + l.off_tab);
+  int4* ex = reinterpret_cast<int4*>(smem_raw + l.off_ex);
+  S* px = reinterpret_cast<S*>(smem_raw + l.off_px);
+  S* mono = reinterpret_cast<S*>(smem_raw + l.off_mono);
+  S* A = reinterpret_cast<S*>(smem_raw + l.off_a);
+  S* B = reinterpret_cast<S*>(smem_raw + l.off_b);
+  const int n = d.n, nm = d.nm, p = d.p, NP = l.NP, KP = l.KP, JC = l.JC;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int KT = KP / kRK;
+  const int at = tid / KT, kt = tid % KT;
+  const bool active = at < NP / kRA;
   const int leaf = blockIdx.x;
+  const int kc0 = blockIdx.y * KC, KCn = min(KC, K - kc0);
   const int2 range = d.leaf_src[leaf];
-  const int n = d.n, nm = d.nm;
-  const int total = n * K;
-  const int base = blockIdx.y * kThreads * kOutputsPerThread;
   const int64_t node = level_offset(d.L) + leaf;
-  S acc[kOutputsPerThread];
+  for (int i = tid; i < n; i += T) tab[i] = d.p2m_tab[i];
+  for (int i = tid; i < nm; i += T) ex[i] = d.p2m_tab[n + i];
+  S acc[kRA][kRK];
 #pragma unroll
-  for (int r = 0; r < kOutputsPerThread; ++r) acc[r] = S(0);
+  for (int i = 0; i < kRA; ++i)
+#pragma unroll
+    for (int j = 0; j < kRK; ++j) acc[i][j] = S(0);
   const auto* rel = static_cast<const typename Vec4<S>::type*>(d.src_rel);
   const S* sm = static_cast<const S*>(d.sm);
-  for (int j0 = range.x; j0 < range.y; j0 += chunk) {
-    const int cnt = min(chunk, range.y - j0);
-    for (int w = threadIdx.x; w < cnt * nm; w += kThreads) {
-      const int j = w / nm, g = w % nm;
-      const int4 e = d.p2m_tab[n + g];  // exponents of basis entry g (second table block)
-      const auto r = rel[j0 + j];
-      mono[j * nm + g] = axis_power<S>(r.x, e.x) * axis_power<S>(r.y, e.y) * axis_power<S>(r.z, e.z);
-    }
-    for (int w = threadIdx.x; w < cnt * 3 * K; w += kThreads) mom[w] = sm[int64_t(j0) * 3 * K + w];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kOutputsPerThread; ++r) {
-      const int o = base + r * kThreads + threadIdx.x;
-      if (o >= total) break;
-      const int alpha = o / K, k = o % K;
-      const int4 t = d.p2m_tab[alpha];
-      S a = acc[r];
-      for (int j = 0; j < cnt; ++j) {
-        const S* mj = mom + j * 3 * K + k;
-        const S* pj = mono + j * nm;
-        if (t.x >= 0) a += pj[t.x] * mj[0];
-        if (t.y >= 0) a += pj[t.y] * mj[K];
-        if (t.z >= 0) a += pj[t.z] * mj[2 * K];
+  for (int j0 = range.x; j0 < range.y; j0 += JC) {
+    const int cnt = min(JC, range.y - j0);
+    __syncthreads();  // previous chunk's products done; tables visible
+    if (tid < 3 * JC) {
+      const int j = tid / 3, a = tid % 3;
+      S x = S(0);
+      if (j < cnt) {
+        const auto r = rel[j0 + j];
+        x = a == 0 ? r.x : (a == 1 ? r.y : r.z);
       }
-      acc[r] = a;
+      S* row = px + (j * 3 + a) * p;
+      for (int i = 0; i < p; ++i) row[i] = axis_power<S>(x, i);
+    }
+    for (int w = tid; w < 3 * JC * KP; w += T) {
+      const int row = w / KP, k = w - row * KP;
+      const int j = row / 3, c = row - 3 * j;
+      B[w] = (j < cnt && k < KCn) ? sm[(int64_t(j0 + j) * 3 + c) * K + kc0 + k] : S(0);
     }
     __syncthreads();
-  }
-  S* M = static_cast<S*>(d.M) + node * total;
-#pragma unroll
-  for (int r = 0; r < kOutputsPerThread; ++r) {
-    const int o = base + r * kThreads + threadIdx.x;
-    if (o >= total) break;
-    const int sign = d.p2m_tab[o / K].w;
-    M[o] = sign > 0 ? acc[r] : -acc[r];
-  }
-}
-
-// Parents of one level; output (alpha, k) gathers all 8 children through the
-// shared CSR pattern of the M2M operator.
-template <class S>
-__global__ void __launch_bounds__(kThreads) m2m_kernel(const DevicePlan d, int level, int K) {
-  const int parent = blockIdx.x;
-  const int n = d.n, total = n * K;
-  const int64_t pnode = level_offset(level) + parent;
-  const int64_t cnode0 = level_offset(level + 1) + int64_t(parent) * 8;
-  const S* M = static_cast<const S*>(d.M);
-  const S* vals = static_cast<const S*>(d.m2m.values) + int64_t(level + 1) * 8 * d.m2m.nnz;
-  S* out = static_cast<S*>(d.M) + pnode * total;
-  const int base = blockIdx.y * kThreads * kOutputsPerThread;
-#pragma unroll
-  for (int r = 0; r < kOutputsPerThread; ++r) {
-    const int o = base + r * kThreads + threadIdx.x;
-    if (o >= total) break;
-    const int alpha = o / K, k = o % K;
-    const int e0 = d.m2m.row_offsets[alpha], e1 = d.m2m.row_offsets[alpha + 1];
-    S a = S(0);
-    for (int c = 0; c < 8; ++c) {
-      const S* child = M + (cnode0 + c) * total + k;
-      const S* v = vals + c * d.m2m.nnz;
-      for (int e = e0; e < e1; ++e) a += v[e] * child[d.m2m.columns[e] * K];
+    for (int g = tid; g < nm; g += T) {
+      const int4 e = ex[g];
+      for (int j = 0; j < JC; ++j) {
+        const S* pj = px + j * 3 * p;
+        mono[j * nm + g] = pj[e.x] * pj[p + e.y] * pj[2 * p + e.z];
+      }
     }
-    out[o] = a;
+    __syncthreads();
+    for (int alpha = tid; alpha < NP; alpha += T) {
+      const int4 t = alpha < n ? tab[alpha] : int4{-1, -1, -1, 0};
+      for (int j = 0; j < JC; ++j) {
+        const S* mj = mono + j * nm;
+        const bool ok = j < cnt;
+        A[(3 * j + 0) * NP + alpha] = ok && t.x >= 0 ? mj[t.x] : S(0);
+        A[(3 * j + 1) * NP + alpha] = ok && t.y >= 0 ? mj[t.y] : S(0);
+        A[(3 * j + 2) * NP + alpha] = ok && t.z >= 0 ? mj[t.z] : S(0);
+      }
+    }
+    __syncthreads();
+    if (active) tile_gemm<S>(A + at * kRA, NP, B + kt * kRK, KP, 3 * min(cnt, JC), acc);
   }
-}
-
-// Children of one level: L_child += L2L(L_parent).
-template <class S>
-__global__ void __launch_bounds__(kThreads) l2l_kernel(const DevicePlan d, int level, int K) {
-  const int child = blockIdx.x;
-  const int n = d.n, total = n * K;
-  const int cls = child & 7;
-  const int64_t cnode = level_offset(level) + child;
-  const int64_t pnode = level_offset(level - 1) + (child >> 3);
-  const S* P = static_cast<const S*>(d.Lc) + pnode * total;
-  S* out = static_cast<S*>(d.Lc) + cnode * total;
-  const S* v = static_cast<const S*>(d.l2l.values) + (int64_t(level) * 8 + cls) * d.l2l.nnz;
-  const int base = blockIdx.y * kThreads * kOutputsPerThread;
+  if (!active) return;
+  S* M = static_cast<S*>(d.M) + node * int64_t(n) * K + kc0;
 #pragma unroll
-  for (int r = 0; r < kOutputsPerThread; ++r) {
-    const int o = base + r * kThreads + threadIdx.x;
-    if (o >= total) break;
-    const int beta = o / K, k = o % K;
-    S a = S(0);
-    for (int e = d.l2l.row_offsets[beta]; e < d.l2l.row_offsets[beta + 1]; ++e)
-      a += v[e] * P[d.l2l.columns[e] * K + k];
-    out[o] += a;
+  for (int i = 0; i < kRA; ++i) {
+    const int alpha = at * kRA + i;
+    if (alpha >= n) break;
+    const S sign = d.p2m_tab[alpha].w > 0 ? S(1) : S(-1);
+#pragma unroll
+    for (int j = 0; j < kRK; ++j) {
+      const int k = kt * kRK + j;
+      if (k < KCn) M[int64_t(alpha) * K + k] = sign * acc[i][j];
+    }
   }
 }
 
-// One workgroup per (target leaf, chunk of kc moment states): far field from
-// the leaf local, plus the near field, written to out[k][user target][3].
+// Dense transposed shift operators: T[(level*8 + class)][col][NP], row
+// (output coefficient) contiguous, rows n..NP-1 zero.
 template <class S>
-__global__ void __launch_bounds__(kThreads) l2p_kernel(const DevicePlan d, S* __restrict__ out, int K, int kc,
-                                                       int tpass) {
-  extern __shared__ unsigned char smem_raw[];
-  const int n = d.n, nm = d.nm;
-  S* loc = reinterpret_cast<S*>(smem_raw);  // n x kc
-  S* mono = loc + n * kc;                    // tpass x nm
+__device__ __forceinline__ const S* shift_operator(const void* dense, int level, int cls, int n, int NP) {
+  return static_cast<const S*>(dense) + (size_t(level) * 8 + cls) * size_t(n) * NP;
+}
+
+// Copy columns [0, KCn) of box coefficients C[n][ld] (global; C already
+// offset to the first column) into LDS B[n][KP], zero-padded.
+template <class S>
+__device__ __forceinline__ void stage_box(S* __restrict__ B, const S* __restrict__ C, int n, int ld, int KCn,
+                                          int KP) {
+  const int tid = threadIdx.x, T = blockDim.x;
+  constexpr int V = 16 / sizeof(S);
+  if (KCn == KP && ld % V == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) {
+    using V4 = typename std::conditional<sizeof(S) == 4, float4, double2>::type;
+    const int vpr = KP / V;  // vectors per row
+    for (int w = tid; w < n * vpr; w += T) {
+      const int row = w / vpr, v = w - row * vpr;
+      reinterpret_cast<V4*>(B)[w] = *reinterpret_cast<const V4*>(C + size_t(row) * ld + v * V);
+    }
+  } else {
+    for (int w = tid; w < n * KP; w += T) {
+      const int row = w / KP, k = w - row * KP;
+      B[w] = k < KCn ? C[size_t(row) * ld + k] : S(0);
+    }
+  }
+}
+
+// One workgroup per parent at `level`: M_parent = sum_c T_c M_child(c).
+template <class S>
+__global__ void __launch_bounds__(1024) m2m_kernel(const DevicePlan d, int level, int K, int KC, int KP) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  S* B = reinterpret_cast<S*>(smem_raw);  // [n][KP]
+  const int n = d.n, NP = round_up(n, kRA), KT = KP / kRK;
+  const int tid = threadIdx.x;
+  const int at = tid / KT, kt = tid % KT;
+  const bool active = at < NP / kRA;
+  const int64_t parent = blockIdx.x;
+  const int kc0 = blockIdx.y * KC, KCn = min(KC, K - kc0);
+  const int64_t cnode0 = level_offset(level + 1) + parent * 8;
+  const S* M = static_cast<const S*>(d.M) + kc0;
+  S acc[kRA][kRK];
+#pragma unroll
+  for (int i = 0; i < kRA; ++i)
+#pragma unroll
+    for (int j = 0; j < kRK; ++j) acc[i][j] = S(0);
+  for (int c = 0; c < 8; ++c) {
+    __syncthreads();
+    stage_box<S>(B, M + (cnode0 + c) * int64_t(n) * K, n, K, KCn, KP);
+    __syncthreads();
+    if (active) tile_gemm<S>(shift_operator<S>(d.m2m_dense, level + 1, c, n, NP) + at * kRA, NP, B + kt * kRK, KP, n, acc);
+  }
+  if (active)
+    store_tile<S, false>(static_cast<S*>(d.M) + (level_offset(level) + parent) * int64_t(n) * K + kc0, at * kRA, n,
+                         kt * kRK, KCn, K, acc);
+}
+
+// One workgroup per parent at child_level - 1: L_child(c) += T_c L_parent.
+template <class S>
+__global__ void __launch_bounds__(1024) l2l_kernel(const DevicePlan d, int child_level, int K, int KC, int KP) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  S* B = reinterpret_cast<S*>(smem_raw);  // [n][KP]
+  const int n = d.n, NP = round_up(n, kRA), KT = KP / kRK;
+  const int tid = threadIdx.x;
+  const int at = tid / KT, kt = tid % KT;
+  const bool active = at < NP / kRA;
+  const int64_t parent = blockIdx.x;
+  const int kc0 = blockIdx.y * KC, KCn = min(KC, K - kc0);
+  S* Lc = static_cast<S*>(d.Lc) + kc0;
+  stage_box<S>(B, Lc + (level_offset(child_level - 1) + parent) * int64_t(n) * K, n, K, KCn, KP);
+  __syncthreads();
+  if (!active) return;
+  for (int c = 0; c < 8; ++c) {
+    S acc[kRA][kRK];
+#pragma unroll
+    for (int i = 0; i < kRA; ++i)
+#pragma unroll
+      for (int j = 0; j < kRK; ++j) acc[i][j] = S(0);
+    tile_gemm<S>(shift_operator<S>(d.l2l_dense, child_level, c, n, NP) + at * kRA, NP, B + kt * kRK, KP, n, acc);
+    const int64_t child = level_offset(child_level) + parent * 8 + c;
+    store_tile<S, true>(Lc + child * int64_t(n) * K, at * kRA, n, kt * kRK, KCn, K, acc);
+  }
+}
+
+constexpr int kL2PTargets = 32;  // targets per chunk
+constexpr int kL2PRT = 2;        // targets per thread tile
+
+// L2P layout: tab[n] int4, ex[nm] int4, px[TC][3][p], monoT[nm][TC],
+// Lb[n][KP] (the leaf local).
+template <class S>
+MonoLayout l2p_layout(const DevicePlan& d, int KC) {
+  MonoLayout l{};
+  l.NP = round_up(d.n, kRA);
+  l.KP = round_up(KC, kRK);
+  l.TCP = kL2PTargets;
+  size_t o = 0;
+  l.off_tab = o;
+  o += align16(size_t(d.n) * sizeof(int4));
+  l.off_ex = o;
+  o += align16(size_t(std::max(d.nm, 1)) * sizeof(int4));
+  l.off_px = o;
+  o += align16(size_t(l.TCP) * 3 * std::max(d.p, 1) * sizeof(S));
+  l.off_mono = o;
+  o += align16(size_t(std::max(d.nm, 1)) * l.TCP * sizeof(S));
+  l.off_b = o;
+  o += align16(size_t(d.n) * l.KP * sizeof(S));
+  l.bytes = o;
+  return l;
+}
+
+// One workgroup per target leaf: far field from the leaf local,
+//   H[t][c][k] = -sum_b L[b][k] d_t^(b-e_c)/(b-e_c)!,
+// a thread owning 2 targets x 4 states x 3 components; plus the near field,
+// written to out[k][user target][3].
+template <class S>
+__global__ void __launch_bounds__(1024) l2p_kernel(const DevicePlan d, S* __restrict__ out, int K, int KC,
+                                                   MonoLayout l) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int4* tab = reinterpret_cast<int4*>(smem_raw + l.off_tab);
+  int4* ex = reinterpret_cast<int4*>(smem_raw + l.off_ex);
+  S* px = reinterpret_cast<S*>(smem_raw + l.off_px);
+  S* monoT = reinterpret_cast<S*>(smem_raw + l.off_mono);
+  S* Lb = reinterpret_cast<S*>(smem_raw + l.off_b);
+  const int n = d.n, nm = d.nm, p = d.p, KP = l.KP;
+  constexpr int TC = kL2PTargets;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int KT = KP / kRK;
+  const int tt = tid / KT, kt = tid % KT;
   const int leaf = blockIdx.x;
-  const int k0 = blockIdx.y * kc;
+  const int kc0 = blockIdx.y * KC, KCn = min(KC, K - kc0);
   const int2 range = d.leaf_tgt[leaf];
   if (range.x == range.y) return;
   const int64_t node = level_offset(d.L) + leaf;
-  const S* Lg = static_cast<const S*>(d.Lc) + node * int64_t(n) * K;
-  for (int w = threadIdx.x; w < n * kc; w += kThreads) {
-    const int b = w / kc, k = k0 + w % kc;
-    loc[w] = k < K ? Lg[b * K + k] : S(0);
-  }
+  for (int i = tid; i < n; i += T) tab[i] = d.p2m_tab[i];
+  for (int i = tid; i < nm; i += T) ex[i] = d.p2m_tab[n + i];
+  stage_box<S>(Lb, static_cast<const S*>(d.Lc) + node * int64_t(n) * K + kc0, n, K, KCn, KP);
   const auto* rel = static_cast<const typename Vec4<S>::type*>(d.tgt_rel);
   const S* Hn = static_cast<const S*>(d.Hn);
-  for (int t0 = range.x; t0 < range.y; t0 += tpass) {
-    const int cnt = min(tpass, range.y - t0);
+  const int k0 = kt * kRK, tl = tt * kL2PRT;
+  for (int t0 = range.x; t0 < range.y; t0 += TC) {
+    const int cnt = min(TC, range.y - t0);
     __syncthreads();
-    for (int w = threadIdx.x; w < cnt * nm; w += kThreads) {
-      const int t = w / nm, g = w % nm;
-      const int4 e = d.p2m_tab[n + g];
-      const auto r = rel[t0 + t];
-      mono[t * nm + g] = axis_power<S>(r.x, e.x) * axis_power<S>(r.y, e.y) * axis_power<S>(r.z, e.z);
+    if (tid < 3 * TC) {
+      const int t = tid / 3, a = tid % 3;
+      S x = S(0);
+      if (t < cnt) {
+        const auto r = rel[t0 + t];
+        x = a == 0 ? r.x : (a == 1 ? r.y : r.z);
+      }
+      S* row = px + (t * 3 + a) * p;
+      for (int i = 0; i < p; ++i) row[i] = axis_power<S>(x, i);
     }
     __syncthreads();
-    const int t = threadIdx.x / kc, kk = threadIdx.x % kc, k = k0 + kk;
-    if (t < cnt && k < K) {
-      const S* pm = mono + t * nm;
-      S hx = S(0), hy = S(0), hz = S(0);
-      for (int b = 0; b < n; ++b) {
-        const int4 tb = d.p2m_tab[b];
-        const S lv = loc[b * kc + kk];
-        if (tb.x >= 0) hx -= lv * pm[tb.x];
-        if (tb.y >= 0) hy -= lv * pm[tb.y];
-        if (tb.z >= 0) hz -= lv * pm[tb.z];
+    for (int w = tid; w < nm * TC; w += T) {
+      const int g = w / TC, t = w - g * TC;
+      const int4 e = ex[g];
+      const S* pt = px + t * 3 * p;
+      monoT[w] = pt[e.x] * pt[p + e.y] * pt[2 * p + e.z];
+    }
+    __syncthreads();
+    if (tl >= cnt) continue;
+    S acc[3][kL2PRT][kRK];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int i = 0; i < kL2PRT; ++i)
+#pragma unroll
+        for (int j = 0; j < kRK; ++j) acc[c][i][j] = S(0);
+    for (int b = 0; b < n; ++b) {
+      const int4 t4 = tab[b];
+      S lv[kRK];
+      load_vec<S, kRK>(Lb + b * KP + k0, lv);
+      const int idx[3] = {t4.x, t4.y, t4.z};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (idx[c] < 0) continue;
+        const S* mrow = monoT + idx[c] * TC + tl;
+        S mv[kL2PRT];
+#pragma unroll
+        for (int i = 0; i < kL2PRT; ++i) mv[i] = mrow[i];
+#pragma unroll
+        for (int i = 0; i < kL2PRT; ++i)
+#pragma unroll
+          for (int j = 0; j < kRK; ++j) acc[c][i][j] = fma(mv[i], lv[j], acc[c][i][j]);
       }
-      const int ts = t0 + t;
-      const S* hn = Hn + int64_t(ts) * 3 * K + k;
-      S* o = out + (int64_t(k) * d.nt + d.tgt_perm[ts]) * 3;
-      o[0] = hx + hn[0];
-      o[1] = hy + hn[K];
-      o[2] = hz + hn[2 * K];
+    }
+#pragma unroll
+    for (int i = 0; i < kL2PRT; ++i) {
+      if (tl + i >= cnt) break;
+      const int ts = t0 + tl + i;
+      const int user = d.tgt_perm[ts];
+      const S* hn = Hn + int64_t(ts) * 3 * K;
+#pragma unroll
+      for (int j = 0; j < kRK; ++j) {
+        if (k0 + j >= KCn) break;
+        const int k = kc0 + k0 + j;
+        S* o = out + (int64_t(k) * d.nt + user) * 3;
+        o[0] = hn[k] - acc[0][i][j];
+        o[1] = hn[K + k] - acc[1][i][j];
+        o[2] = hn[2 * K + k] - acc[2][i][j];
+      }
     }
   }
 }
@@ -210,8 +446,12 @@ __global__ void zero_kernel(S* p, size_t count) {
     p[i] = S(0);
 }
 
-inline unsigned groups_for(int total) {
-  return unsigned((total + kThreads * kOutputsPerThread - 1) / (kThreads * kOutputsPerThread));
+// Threads for a tile grid of (row tiles) x (state tiles), whole waves.
+inline unsigned tile_threads(int row_tiles, int KP) {
+  const int t = row_tiles * (KP / kRK);
+  const int threads = std::max(64, round_up(t, 64));
+  if (threads > 1024) throw invalid_argument("expansion order and batch too large for one workgroup per box");
+  return unsigned(threads);
 }
 
 }  // namespace
@@ -225,39 +465,54 @@ void launch_gather(const DevicePlan& d, const S* user_moments, S scale, int K, h
   HIPFMM_CHECK(hipGetLastError());
 }
 
+// Widest state chunk (a multiple of 4, or K) whose LDS layout fits 64 KiB.
+template <class F>
+int state_chunk(int K, F bytes_for) {
+  int kc = K;
+  while (kc > kRK && bytes_for(kc) > 65536) kc = round_up(kc / 2, kRK);
+  if (bytes_for(kc) > 65536) throw invalid_argument("expansion order too large for the box kernels' LDS");
+  return kc;
+}
+
 template <class S>
 void launch_p2m(const DevicePlan& d, int K, hipStream_t s) {
-  const int chunk = sizeof(S) == 4 ? 32 : 16;
-  const size_t lds = size_t(chunk) * (d.nm + 3 * K) * sizeof(S);
-  const dim3 grid(unsigned(d.leaves), groups_for(d.n * K));
-  p2m_kernel<S><<<grid, kThreads, lds, s>>>(d, K, chunk);
+  if (d.leaves == 0) return;
+  const int kc = state_chunk(K, [&](int c) { return p2m_layout<S>(d, c).bytes; });
+  const MonoLayout l = p2m_layout<S>(d, kc);
+  const unsigned threads = std::max(tile_threads(l.NP / kRA, l.KP), unsigned(round_up(3 * l.JC, 64)));
+  const dim3 grid(unsigned(d.leaves), unsigned((K + kc - 1) / kc));
+  p2m_kernel<S><<<grid, threads, l.bytes, s>>>(d, K, kc, l);
   HIPFMM_CHECK(hipGetLastError());
 }
 
 template <class S>
 void launch_m2m(const DevicePlan& d, int parent_level, int K, hipStream_t s) {
-  const dim3 grid(unsigned(int64_t(1) << (3 * parent_level)), groups_for(d.n * K));
-  m2m_kernel<S><<<grid, kThreads, 0, s>>>(d, parent_level, K);
+  const int NP = round_up(d.n, kRA);
+  const int kc = state_chunk(K, [&](int c) { return size_t(d.n) * round_up(c, kRK) * sizeof(S); });
+  const int KP = round_up(kc, kRK);
+  const dim3 grid(unsigned(int64_t(1) << (3 * parent_level)), unsigned((K + kc - 1) / kc));
+  m2m_kernel<S><<<grid, tile_threads(NP / kRA, KP), size_t(d.n) * KP * sizeof(S), s>>>(d, parent_level, K, kc, KP);
   HIPFMM_CHECK(hipGetLastError());
 }
 
 template <class S>
 void launch_l2l(const DevicePlan& d, int child_level, int K, hipStream_t s) {
-  const dim3 grid(unsigned(int64_t(1) << (3 * child_level)), groups_for(d.n * K));
-  l2l_kernel<S><<<grid, kThreads, 0, s>>>(d, child_level, K);
+  const int NP = round_up(d.n, kRA);
+  const int kc = state_chunk(K, [&](int c) { return size_t(d.n) * round_up(c, kRK) * sizeof(S); });
+  const int KP = round_up(kc, kRK);
+  const dim3 grid(unsigned(int64_t(1) << (3 * (child_level - 1))), unsigned((K + kc - 1) / kc));
+  l2l_kernel<S><<<grid, tile_threads(NP / kRA, KP), size_t(d.n) * KP * sizeof(S), s>>>(d, child_level, K, kc, KP);
   HIPFMM_CHECK(hipGetLastError());
 }
 
 template <class S>
 void launch_l2p_unsort(const DevicePlan& d, S* fields, int K, hipStream_t s) {
-  // Stage at most 32 KB of the leaf local per workgroup.
-  int kc = K;
-  while (kc > 1 && size_t(d.n) * kc * sizeof(S) > 32768) kc = (kc + 1) / 2;
-  // Targets per pass: every thread busy, monomial table within 24 KB.
-  const int tpass = std::max(1, std::min(kThreads / kc, int(24576 / (std::max(d.nm, 1) * sizeof(S)))));
-  const size_t lds = (size_t(d.n) * kc + size_t(tpass) * d.nm) * sizeof(S);
+  if (d.leaves == 0) return;
+  const int kc = state_chunk(K, [&](int c) { return l2p_layout<S>(d, c).bytes; });
+  const MonoLayout l = l2p_layout<S>(d, kc);
+  const unsigned threads = std::max(tile_threads(l.TCP / kL2PRT, l.KP), unsigned(round_up(3 * l.TCP, 64)));
   const dim3 grid(unsigned(d.leaves), unsigned((K + kc - 1) / kc));
-  l2p_kernel<S><<<grid, kThreads, lds, s>>>(d, fields, K, kc, tpass);
+  l2p_kernel<S><<<grid, threads, l.bytes, s>>>(d, fields, K, kc, l);
   HIPFMM_CHECK(hipGetLastError());
 }
 

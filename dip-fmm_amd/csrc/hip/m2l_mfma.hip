@@ -67,9 +67,24 @@ constexpr int wait_vm_lgkm0(int vm) { return (vm & 15) | ((vm >> 4) << 14) | (7 
 // instructions, which the kernel does not contain).
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-__device__ __forceinline__ void lds_dma16(const uint4* src, uint4* lds_wave_base) {
-  const uint32_t lds = uint32_t(uintptr_t((__attribute__((address_space(3))) uint4*)lds_wave_base));
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds) : "memory", "m0");
+__device__ __forceinline__ uint32_t lds_address(const uint4* p) {
+  return uint32_t(uintptr_t((const __attribute__((address_space(3))) uint4*)p));
+}
+// Per-lane 64-bit source address.
+__device__ __forceinline__ void lds_dma16(const uint4* src, const uint4* lds_wave_base) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src),
+               "s"(lds_address(lds_wave_base))
+               : "memory", "m0");
+}
+// Wave-uniform base plus a per-lane 32-bit byte offset.  Callers pass a base
+// that is the same on every lane; it is moved to an SGPR pair explicitly.
+__device__ __forceinline__ void lds_dma16_sbase(const void* base, uint32_t lane_bytes, const uint4* lds_wave_base) {
+  const uint64_t b = uint64_t(uintptr_t(base));
+  const uint64_t sb = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b))))) |
+                      (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b >> 32))))) << 32);
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(lane_bytes), "s"(sb),
+               "s"(lds_address(lds_wave_base))
+               : "memory", "m0");
 }
 #pragma clang diagnostic pop
 
@@ -116,7 +131,6 @@ __global__ void __launch_bounds__(256) split_kernel(const DevicePlan d, int K, i
   }
 }
 
-__device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 struct LevelTable {
   int count;
@@ -128,7 +142,9 @@ struct LevelTable {
   int kchunks[10];
 };
 
-// Per-lane view of the columns of the NB tiles one wave owns.
+// Per-lane view of the columns of the NB tiles one wave owns.  With
+// kUniformBox (32 states per box, K a multiple of 32) every column of a tile
+// belongs to the same box, so the box coordinates are wave-uniform.
 template <int NB>
 struct Columns {
   bool valid[NB];
@@ -136,12 +152,12 @@ struct Columns {
 };
 
 template <int NB>
-__device__ __forceinline__ bool source_inside(const Columns<NB>& c, int nb, int4 o, int nbox, int& sx, int& sy,
-                                              int& sz) {
-  sx = c.tx[nb] - o.x;
-  sy = c.ty[nb] - o.y;
-  sz = c.tz[nb] - o.z;
-  return c.valid[nb] && sx >= 0 && sy >= 0 && sz >= 0 && sx < nbox && sy < nbox && sz < nbox;
+__device__ __forceinline__ bool source_inside(const Columns<NB>& c, int nb, int ox, int oy, int oz, int nbox, int& sx,
+                                              int& sy, int& sz) {
+  sx = c.tx[nb] - ox;
+  sy = c.ty[nb] - oy;
+  sz = c.tz[nb] - oz;
+  return c.valid[nb] & (sx >= 0) & (sy >= 0) & (sz >= 0) & (sx < nbox) & (sy < nbox) & (sz < nbox);
 }
 
 struct StepGeometry {
@@ -149,78 +165,114 @@ struct StepGeometry {
   int64_t lvl0;
 };
 
-// Fetch the operands of step `step` (clamped to the last step, so every
-// step issues the same number of DMA operations per wave) into one ring slot.
-template <int MT, int NB, int W>
-__device__ __forceinline__ void issue_step(int step, uint4* a_slot, uint4* b_slot, const int4* offs,
-                                           const DevicePlan& d, const Columns<NB>& c, const StepGeometry& sg,
-                                           int wave, int lane) {
-  constexpr int kAChunks = 2 * MT, kAPerWave = (kAChunks + W - 1) / W;
-  const int st = step < sg.nsteps ? step : sg.nsteps - 1;
-  const int ks = st / kParityOffsets, oi = st - kParityOffsets * ks;
-  const int4 o = offs[oi];
-  const int64_t tbase = (int64_t(o.w) * sg.KS + ks) * MT * 64 + lane;
-#pragma unroll
-  for (int i = 0; i < kAPerWave; ++i) {
-    int ch = wave + W * i;
-    if (ch >= kAChunks) ch -= kAChunks;  // repeats another wave's chunk: same bytes to the same place
-    const int half = ch >= MT ? 1 : 0, m = ch - half * MT;
-    const uint4* src = static_cast<const uint4*>(half ? d.T_lo : d.T_hi) + tbase + int64_t(m) * 64;
-    lds_dma16(src, a_slot + ch * 64);
-  }
-  const uint4* zero = static_cast<const uint4*>(d.zero_block) + lane;
-  const uint4* mhi = static_cast<const uint4*>(d.mt_hi);
-  const uint4* mlo = static_cast<const uint4*>(d.mt_lo);
-  const int g = lane >> 5;
-#pragma unroll
-  for (int nb = 0; nb < NB; ++nb) {
-    int sx, sy, sz;
-    const bool inside = source_inside(c, nb, o, sg.nbox, sx, sy, sz);
-    // Branch-free: the Morton index of an outside box is computed and discarded.
-    const int64_t snode = sg.lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), sg.level);
-    const int64_t at = ((snode * sg.KS + ks) * sg.K + c.kk[nb]) * 2 + g;
-    uint4* dst = b_slot + (wave * NB + nb) * 128;
-    lds_dma16(inside ? mhi + at : zero, dst);
-    lds_dma16(inside ? mlo + at : zero, dst + 64);
-  }
+// Per-workgroup step tables of the uniform-box path, built once: for each
+// used offset, the byte offset of its class-matrix block and, per tile, the
+// byte offset of the source box's multipole block (-1: outside the grid).
+template <int W, int NB>
+struct StepTables {
+  long long a[kParityOffsets];
+  long long b[kParityOffsets][W * NB];
+};
+
+__device__ __forceinline__ long long uniform_i64(long long v) {
+  const uint64_t u = uint64_t(v);
+  return (long long)(uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u))))) |
+                     (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u >> 32))))) << 32));
 }
 
-// The 3*MT*NB products of one step from a ring slot; a wave whose columns
-// all lie outside the grid for this offset skips them.
-template <int MT, int NB>
-__device__ __forceinline__ void compute_step(int step, const uint4* a_slot, const uint4* b_slot, const int4* offs,
-                                             const Columns<NB>& c, const StepGeometry& sg, int wave, int lane,
-                                             f32x16 (&acc)[NB][MT]) {
-  const int oi = step % kParityOffsets;
-  const int4 o = offs[oi];
-  bool any = false;
+// Fetch the operands of step (ks, offset oi) into one ring slot.  Every step
+// issues the same number of DMA operations per wave.
+template <int MT, int NB, int W, bool kUniformBox>
+__device__ __forceinline__ void issue_step(int ks, int oi, const uint4* a_slot, const uint4* b_slot,
+                                           const int4* offs, const StepTables<W, NB>& tab, const DevicePlan& d,
+                                           const Columns<NB>& c, const StepGeometry& sg, int wave, int lane) {
+  constexpr int kAChunks = 2 * MT, kAPerWave = (kAChunks + W - 1) / W;
+  const int g = lane >> 5, col = lane & 31;
+  if constexpr (kUniformBox) {
+    const long long ta = uniform_i64(tab.a[oi]) + (long long)ks * MT * 1024;
 #pragma unroll
-  for (int nb = 0; nb < NB; ++nb) {
-    int sx, sy, sz;
-    any |= source_inside(c, nb, o, sg.nbox, sx, sy, sz);
-  }
-  if (!__any(any)) return;
-  bf16x8 bh[NB], bl[NB];
-#pragma unroll
-  for (int nb = 0; nb < NB; ++nb) {
-    const uint4* src = b_slot + (wave * NB + nb) * 128 + lane;
-    bh[nb] = __builtin_bit_cast(bf16x8, src[0]);
-    bl[nb] = __builtin_bit_cast(bf16x8, src[64]);
-  }
-#pragma unroll
-  for (int m = 0; m < MT; ++m) {
-    const bf16x8 ah = __builtin_bit_cast(bf16x8, a_slot[m * 64 + lane]);
-    const bf16x8 al = __builtin_bit_cast(bf16x8, a_slot[(MT + m) * 64 + lane]);
+    for (int i = 0; i < kAPerWave; ++i) {
+      int ch = wave + W * i;
+      if (ch >= kAChunks) ch -= kAChunks;  // repeats another wave's chunk: same bytes to the same place
+      const int half = ch >= MT ? 1 : 0, m = ch - half * MT;
+      const char* base = static_cast<const char*>(half ? d.T_lo : d.T_hi) + ta + m * 1024;
+      lds_dma16_sbase(base, uint32_t(lane) * 16u, a_slot + ch * 64);
+    }
+    const long long kstep = (long long)ks * sg.K * 32;
+    const uint32_t lane_bytes = uint32_t(col * 2 + g) * 16u;
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[nb], acc[nb][m], 0, 0, 0);
-      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[nb], acc[nb][m], 0, 0, 0);
-      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[nb], acc[nb][m], 0, 0, 0);
+      const long long tb = uniform_i64(tab.b[oi][wave * NB + nb]);
+      const bool inside = tb >= 0;
+      const char* bh = inside ? static_cast<const char*>(d.mt_hi) + tb + kstep : static_cast<const char*>(d.zero_block);
+      const char* bl = inside ? static_cast<const char*>(d.mt_lo) + tb + kstep : static_cast<const char*>(d.zero_block);
+      const uint4* dst = b_slot + (wave * NB + nb) * 128;
+      lds_dma16_sbase(bh, lane_bytes, dst);
+      lds_dma16_sbase(bl, lane_bytes, dst + 64);
+    }
+  } else {
+    const int4 o = offs[oi];
+    const int cls = __builtin_amdgcn_readfirstlane(o.w);
+    const int64_t tunits = (int64_t(cls) * sg.KS + ks) * MT * 64;  // uint4 units
+#pragma unroll
+    for (int i = 0; i < kAPerWave; ++i) {
+      int ch = wave + W * i;
+      if (ch >= kAChunks) ch -= kAChunks;
+      const int half = ch >= MT ? 1 : 0, m = ch - half * MT;
+      const uint4* base = static_cast<const uint4*>(half ? d.T_lo : d.T_hi) + tunits + int64_t(m) * 64;
+      lds_dma16_sbase(base, uint32_t(lane) * 16u, a_slot + ch * 64);
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      int sx, sy, sz;
+      const bool inside = source_inside(c, nb, o.x, o.y, o.z, sg.nbox, sx, sy, sz);
+      const int64_t snode = sg.lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), sg.level);
+      const uint4* dst = b_slot + (wave * NB + nb) * 128;
+      const int64_t at = ((snode * sg.KS + ks) * sg.K + c.kk[nb]) * 2 + g;
+      const uint4* zero = static_cast<const uint4*>(d.zero_block) + lane;
+      lds_dma16(inside ? static_cast<const uint4*>(d.mt_hi) + at : zero, dst);
+      lds_dma16(inside ? static_cast<const uint4*>(d.mt_lo) + at : zero, dst + 64);
     }
   }
 }
 
 template <int MT, int NB, int W>
+struct Fragments {
+  uint4 a[2 * MT];  // hi rows tiles, then lo
+  uint4 b[NB][2];   // per tile: hi, lo
+};
+
+template <int MT, int NB, int W>
+__device__ __forceinline__ void read_fragments(const uint4* a_slot, const uint4* b_slot, int wave, int lane,
+                                               Fragments<MT, NB, W>& f) {
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const uint4* src = b_slot + (wave * NB + nb) * 128 + lane;
+    f.b[nb][0] = src[0];
+    f.b[nb][1] = src[64];
+  }
+#pragma unroll
+  for (int i = 0; i < 2 * MT; ++i) f.a[i] = a_slot[i * 64 + lane];
+}
+
+template <int MT, int NB, int W>
+__device__ __forceinline__ void multiply(const Fragments<MT, NB, W>& f, f32x16 (&acc)[NB][MT]) {
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    const bf16x8 ah = __builtin_bit_cast(bf16x8, f.a[m]);
+    const bf16x8 al = __builtin_bit_cast(bf16x8, f.a[MT + m]);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const bf16x8 bh = __builtin_bit_cast(bf16x8, f.b[nb][0]);
+      const bf16x8 bl = __builtin_bit_cast(bf16x8, f.b[nb][1]);
+      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[nb][m], 0, 0, 0);
+      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[nb][m], 0, 0, 0);
+      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[nb][m], 0, 0, 0);
+    }
+  }
+}
+
+template <int MT, int NB, int W, bool kUniformBox>
 __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d, const LevelTable lt, int K) {
   static_assert(kStages == 4, "the step loop below is unrolled for four ring slots");
   constexpr int kAUnits = 2 * MT * 64, kBUnits = W * NB * 2 * 64;
@@ -229,7 +281,11 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
   static_assert((kStages - 2) * kDmaPerStep <= 63, "vmcnt holds at most 63 outstanding operations");
   // Ring slots: the class-matrix slice, then the waves' source fragments.
   __shared__ uint4 r0[kAUnits + kBUnits], r1[kAUnits + kBUnits], r2[kAUnits + kBUnits], r3[kAUnits + kBUnits];
-  __shared__ int4 offs[kParityOffsets];
+  __shared__ int4 offs_all[kParityOffsets];
+  __shared__ int4 offs[kParityOffsets];  // the offsets some column of this workgroup uses
+  __shared__ unsigned char used[W][kParityOffsets];
+  __shared__ int active_count;
+  __shared__ StepTables<W, NB> tab;
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
   int li = lt.count - 1;
   while (li > 0 && int(blockIdx.x) < lt.block_begin[li]) --li;
@@ -243,28 +299,107 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
   sg.KS = d.apad / 16;
   sg.level = level;
   sg.nbox = 1 << level;
-  sg.nsteps = sg.KS * kParityOffsets;
   sg.lvl0 = level_offset(level);
   const int64_t parents = int64_t(1) << (3 * (level - 1));
   const int cpb = lt.cols_per_box[li], kchunks = lt.kchunks[li], tiles = lt.tiles[li];
-  const int boxes_per_tile = 32 / cpb;
   const int col = lane & 31, g = lane >> 5;
   Columns<NB> c;
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
     const int tile = (wgp * W + wave) * NB + nb;
     const int bg = tile / kchunks, kc = tile % kchunks;
-    const int64_t q = int64_t(bg) * boxes_per_tile + col / cpb;
-    c.kk[nb] = kc * cpb + col % cpb;
-    c.valid[nb] = tile < tiles && q < parents && c.kk[nb] < K;
+    int64_t q;
+    if constexpr (kUniformBox) {
+      q = bg;  // one box per tile
+      c.kk[nb] = kc * 32;
+      c.valid[nb] = tile < tiles && q < parents;
+    } else {
+      q = int64_t(bg) * (32 / cpb) + col / cpb;
+      c.kk[nb] = kc * cpb + col % cpb;
+      c.valid[nb] = tile < tiles && q < parents && c.kk[nb] < K;
+    }
     int x = 0, y = 0, z = 0;
     if (c.valid[nb]) demorton3(uint32_t(q), level - 1, x, y, z);
     c.tx[nb] = 2 * x + px;
     c.ty[nb] = 2 * y + py;
     c.tz[nb] = 2 * z + pz;
   }
-  for (int i = threadIdx.x; i < kParityOffsets; i += W * 64) offs[i] = d.parity_offsets[parity * kParityOffsets + i];
+  for (int i = threadIdx.x; i < kParityOffsets; i += W * 64) offs_all[i] = d.parity_offsets[parity * kParityOffsets + i];
   __syncthreads();
+  // Offsets no column of this workgroup can use (source outside the grid for
+  // all of them) are dropped from the step sequence.
+  if constexpr (kUniformBox) {
+    // Every tile is one box: thread i decides offset i for all W*NB tiles.
+    for (int i = threadIdx.x; i < kParityOffsets; i += W * 64) {
+      const int4 o = offs_all[i];
+      bool any = false;
+      for (int t = 0; t < W * NB; ++t) {
+        const int tile = wgp * W * NB + t;
+        const int bg = tile / kchunks;
+        if (tile >= tiles || bg >= parents) continue;
+        int x, y, z;
+        demorton3(uint32_t(bg), level - 1, x, y, z);
+        const int sx = 2 * x + px - o.x, sy = 2 * y + py - o.y, sz = 2 * z + pz - o.z;
+        any |= sx >= 0 && sy >= 0 && sz >= 0 && sx < sg.nbox && sy < sg.nbox && sz < sg.nbox;
+      }
+      used[0][i] = any ? 1 : 0;
+    }
+  } else {
+  for (int i = 0; i < kParityOffsets; ++i) {
+    const int4 o = offs_all[i];
+    bool any = false;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      int sx, sy, sz;
+      any |= source_inside(c, nb, o.x, o.y, o.z, sg.nbox, sx, sy, sz);
+    }
+    const bool wave_any = __any(any) != 0;
+    if (lane == 0) used[wave][i] = wave_any ? 1 : 0;
+  }
+  }
+  __syncthreads();
+  if (wave == 0) {
+    int count = 0;
+    for (int i0 = 0; i0 < kParityOffsets; i0 += 64) {
+      const int i = i0 + lane;
+      bool u = false;
+      if (i < kParityOffsets)
+#pragma unroll
+        for (int w = 0; w < (kUniformBox ? 1 : W); ++w) u |= used[w][i] != 0;
+      const unsigned long long mask = __ballot(u);
+      if (u) offs[count + __popcll(mask & ((1ull << lane) - 1ull))] = offs_all[i];
+      count += __popcll(mask);
+    }
+    if (lane == 0) active_count = count;
+  }
+  __syncthreads();
+  const int nact = active_count;
+  sg.nsteps = sg.KS * nact;
+  if constexpr (kUniformBox) {
+    // Step tables of the used offsets.
+    for (int e = threadIdx.x; e < nact * (W * NB + 1); e += W * 64) {
+      const int a = e / (W * NB + 1), t = e - a * (W * NB + 1);
+      const int4 o = offs[a];
+      if (t == W * NB) {
+        tab.a[a] = (long long)o.w * sg.KS * MT * 1024;
+        continue;
+      }
+      const int tile = wgp * W * NB + t;
+      const int bg = tile / kchunks, kc = tile % kchunks;
+      long long off = -1;
+      if (tile < tiles && bg < parents) {
+        int x, y, z;
+        demorton3(uint32_t(bg), level - 1, x, y, z);
+        const int sx = 2 * x + px - o.x, sy = 2 * y + py - o.y, sz = 2 * z + pz - o.z;
+        if (sx >= 0 && sy >= 0 && sz >= 0 && sx < sg.nbox && sy < sg.nbox && sz < sg.nbox) {
+          const long long snode = sg.lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), level);
+          off = (snode * sg.KS * K + (long long)kc * 32) * 32;
+        }
+      }
+      tab.b[a][t] = off;
+    }
+    __syncthreads();
+  }
 
   f32x16 acc[NB][MT];
 #pragma unroll
@@ -274,29 +409,49 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[nb][m][r] = 0.0f;
 
-  issue_step<MT, NB, W>(0, r0, r0 + kAUnits, offs, d, c, sg, wave, lane);
-  issue_step<MT, NB, W>(1, r1, r1 + kAUnits, offs, d, c, sg, wave, lane);
-  issue_step<MT, NB, W>(2, r2, r2 + kAUnits, offs, d, c, sg, wave, lane);
-  // Step s: wait for this wave's DMA of step s (the two younger steps may
-  // stay in flight) and for its LDS reads of step s-1, barrier (every wave's
-  // share of slot s is in LDS; slot s-1 is free), refill slot s-1 with step
-  // s+3, then multiply out of slot s.  The inline-asm barrier keeps the
-  // compiler from moving LDS accesses across it.
-#define HIPFMM_RING_STEP(S_, CUR, NXT)                                                  \
-  if ((S_) < sg.nsteps) {                                                               \
-    __builtin_amdgcn_s_waitcnt(wait_vm_lgkm0((kStages - 2) * kDmaPerStep));             \
-    asm volatile("s_barrier" ::: "memory");                                             \
-    issue_step<MT, NB, W>((S_) + kStages - 1, r##NXT, r##NXT + kAUnits, offs, d, c, sg, wave, lane); \
-    compute_step<MT, NB>((S_), r##CUR, r##CUR + kAUnits, offs, c, sg, wave, lane, acc);           \
+  if (nact > 0) {
+    // (ks, oi) of the next step to fetch; past the last step the last step
+    // is fetched again into a slot nobody reads.
+    int iks = 0, ioi = 0;
+    auto advance = [&]() {  // branch-free: selects only
+      const bool last = (iks == sg.KS - 1) & (ioi == nact - 1);
+      const bool wrap = ioi + 1 == nact;
+      const int next_oi = wrap ? 0 : ioi + 1;
+      const int next_ks = iks + (wrap ? 1 : 0);
+      ioi = last ? ioi : next_oi;
+      iks = last ? iks : next_ks;
+    };
+    issue_step<MT, NB, W, kUniformBox>(iks, ioi, r0, r0 + kAUnits, offs, tab, d, c, sg, wave, lane);
+    advance();
+    issue_step<MT, NB, W, kUniformBox>(iks, ioi, r1, r1 + kAUnits, offs, tab, d, c, sg, wave, lane);
+    advance();
+    issue_step<MT, NB, W, kUniformBox>(iks, ioi, r2, r2 + kAUnits, offs, tab, d, c, sg, wave, lane);
+    advance();
+    // Step s: wait for this wave's DMA of step s (the two younger steps may
+    // stay in flight) and for its LDS reads of step s-1, barrier (every
+    // wave's share of slot s is in LDS; slot s-1 is free), read the
+    // fragments of slot s, refill slot s-1 with step s+3, and multiply; the
+    // products overlap the next fetch's address arithmetic.  The inline-asm
+    // barrier keeps the compiler from moving LDS accesses across it.
+#define HIPFMM_RING_STEP(S_, CUR, NXT)                                                              \
+  if ((S_) < sg.nsteps) {                                                                           \
+    __builtin_amdgcn_s_waitcnt(wait_vm_lgkm0((kStages - 2) * kDmaPerStep));                         \
+    asm volatile("s_barrier" ::: "memory");                                                         \
+    Fragments<MT, NB, W> f;                                                                         \
+    read_fragments<MT, NB, W>(r##CUR, r##CUR + kAUnits, wave, lane, f);                             \
+    issue_step<MT, NB, W, kUniformBox>(iks, ioi, r##NXT, r##NXT + kAUnits, offs, tab, d, c, sg, wave, lane); \
+    advance();                                                                                      \
+    multiply<MT, NB, W>(f, acc);                                                                    \
   }
-  for (int s = 0; s < sg.nsteps; s += kStages) {
-    HIPFMM_RING_STEP(s, 0, 3)
-    HIPFMM_RING_STEP(s + 1, 1, 0)
-    HIPFMM_RING_STEP(s + 2, 2, 1)
-    HIPFMM_RING_STEP(s + 3, 3, 2)
-  }
+    for (int s = 0; s < sg.nsteps; s += kStages) {
+      HIPFMM_RING_STEP(s, 0, 3)
+      HIPFMM_RING_STEP(s + 1, 1, 0)
+      HIPFMM_RING_STEP(s + 2, 2, 1)
+      HIPFMM_RING_STEP(s + 3, 3, 2)
+    }
 #undef HIPFMM_RING_STEP
-  __builtin_amdgcn_s_waitcnt(0);  // the trailing (clamped) DMAs land before the workgroup's LDS is released
+    __builtin_amdgcn_s_waitcnt(0);  // the trailing repeat fetches land before the workgroup's LDS is released
+  }
 
   const float* lsp = static_cast<const float*>(d.ls_pad) + int64_t(level) * BP;
   float* Lc = static_cast<float*>(d.Lc);
@@ -304,7 +459,7 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
   for (int nb = 0; nb < NB; ++nb) {
     if (!c.valid[nb]) continue;
     const int64_t tnode = sg.lvl0 + morton3(uint32_t(c.tx[nb]), uint32_t(c.ty[nb]), uint32_t(c.tz[nb]), level);
-    float* out = Lc + tnode * int64_t(n) * K + c.kk[nb];
+    float* out = Lc + tnode * int64_t(n) * K + c.kk[nb] + (kUniformBox ? col : 0);
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -343,7 +498,11 @@ LevelTable level_table(const DevicePlan& d, int K) {
 template <int MT>
 void launch_mfma_t(const DevicePlan& d, int K, hipStream_t s) {
   const LevelTable lt = level_table(d, K);
-  m2l_mfma_kernel<MT, kNB, kWaves><<<unsigned(lt.block_begin[lt.count]), kWaves * 64, 0, s>>>(d, lt, K);
+  const unsigned blocks = unsigned(lt.block_begin[lt.count]);
+  if (K % 32 == 0)
+    m2l_mfma_kernel<MT, kNB, kWaves, true><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
+  else
+    m2l_mfma_kernel<MT, kNB, kWaves, false><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
   HIPFMM_CHECK(hipGetLastError());
 }
 

@@ -201,6 +201,22 @@ hipfmm::ShiftCsr shift_csr(hipfmm_plan* P, const double* dense, int L, int n) {
   return out;
 }
 
+// Dense transposed copy of a shift operator family for the box-GEMM kernels:
+// out[(l*8 + c)][col][NP] = op_l,c[row][col], rows padded to NP = round8(n).
+template <class S>
+void* shift_dense_transposed(hipfmm_plan* P, const double* dense, int L, int n) {
+  const int NP = (n + 7) / 8 * 8;
+  std::vector<S> out(size_t(L + 1) * 8 * n * NP, S(0));
+  for (int l = 1; l <= L; ++l)
+    for (int c = 0; c < 8; ++c) {
+      const double* m = dense + (size_t(l - 1) * 8 + c) * n * n;
+      S* o = out.data() + (size_t(l) * 8 + c) * n * NP;
+      for (int r = 0; r < n; ++r)
+        for (int col = 0; col < n; ++col) o[size_t(col) * NP + r] = S(m[size_t(r) * n + col]);
+    }
+  return upload(P, out.data(), out.size());
+}
+
 template <class S>
 void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   DevicePlan& d = P->d;
@@ -329,6 +345,8 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   if (L >= 1) {
     d.m2m = shift_csr<S>(P, desc.m2m_operators, L, n);
     d.l2l = shift_csr<S>(P, desc.l2l_operators, L, n);
+    d.m2m_dense = shift_dense_transposed<S>(P, desc.m2m_operators, L, n);
+    d.l2l_dense = shift_dense_transposed<S>(P, desc.l2l_operators, L, n);
   }
   P->stats.scalar_bytes = sizeof(S);
   P->stats.m2m_unique_matrix_count = uint64_t(L) * 8;
