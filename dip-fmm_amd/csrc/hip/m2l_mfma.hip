@@ -21,7 +21,7 @@
 //   * operands of a step (the class matrix slice T_o[ks], hi and lo, shared
 //     by the workgroup; each wave's source fragments) are fetched straight
 //     into LDS by global_load_lds DMA, kStages - 1 steps ahead, into a ring
-//     of kStages slots held in separate LDS arrays; one barrier per step
+//     of kStages slots; one barrier per step
 //     publishes a slot to every wave and frees the oldest one;
 //   * block b -> parity b % 8, so the 8 XCDs each serve one parity class and
 //     keep its class matrices in their own L2; consecutive workgroups of one
@@ -47,9 +47,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kNB = 2;      // 32-column tiles per wave
-constexpr int kWaves = 4;   // waves per workgroup
-constexpr int kStages = 4;  // LDS ring slots (operands fetched kStages - 1 steps ahead)
+constexpr int kWaves = 4;  // waves per workgroup (one per SIMD)
 constexpr int kParityOffsets = 189;
 
 // s_waitcnt immediate (gfx9 encoding): at most `vm` vector-memory operations
@@ -236,15 +234,17 @@ __device__ __forceinline__ void issue_step(int ks, int oi, const uint4* a_slot, 
   }
 }
 
-template <int MT, int NB, int W>
+// Operand fragments of one step: the wave's source fragments (hi, lo per
+// tile) and the class-matrix row tiles (hi, lo per row tile).
+template <int MT, int NB>
 struct Fragments {
-  uint4 a[2 * MT];  // hi rows tiles, then lo
-  uint4 b[NB][2];   // per tile: hi, lo
+  uint4 a[MT][2];
+  uint4 b[NB][2];
 };
 
-template <int MT, int NB, int W>
+template <int MT, int NB>
 __device__ __forceinline__ void read_fragments(const uint4* a_slot, const uint4* b_slot, int wave, int lane,
-                                               Fragments<MT, NB, W>& f) {
+                                               Fragments<MT, NB>& f) {
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
     const uint4* src = b_slot + (wave * NB + nb) * 128 + lane;
@@ -252,19 +252,24 @@ __device__ __forceinline__ void read_fragments(const uint4* a_slot, const uint4*
     f.b[nb][1] = src[64];
   }
 #pragma unroll
-  for (int i = 0; i < 2 * MT; ++i) f.a[i] = a_slot[i * 64 + lane];
+  for (int m = 0; m < MT; ++m) {
+    f.a[m][0] = a_slot[m * 64 + lane];
+    f.a[m][1] = a_slot[(MT + m) * 64 + lane];
+  }
 }
 
-template <int MT, int NB, int W>
-__device__ __forceinline__ void multiply(const Fragments<MT, NB, W>& f, f32x16 (&acc)[NB][MT]) {
+// The 3 * NB products of row tiles [M0, M1) (hi*hi + hi*lo + lo*hi).
+template <int MT, int NB, int M0, int M1>
+__device__ __forceinline__ void multiply_rows(const uint4 (&a)[MT][2], const uint4 (&b)[NB][2],
+                                              f32x16 (&acc)[NB][MT]) {
 #pragma unroll
-  for (int m = 0; m < MT; ++m) {
-    const bf16x8 ah = __builtin_bit_cast(bf16x8, f.a[m]);
-    const bf16x8 al = __builtin_bit_cast(bf16x8, f.a[MT + m]);
+  for (int m = M0; m < M1; ++m) {
+    const bf16x8 ah = __builtin_bit_cast(bf16x8, a[m][0]);
+    const bf16x8 al = __builtin_bit_cast(bf16x8, a[m][1]);
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      const bf16x8 bh = __builtin_bit_cast(bf16x8, f.b[nb][0]);
-      const bf16x8 bl = __builtin_bit_cast(bf16x8, f.b[nb][1]);
+      const bf16x8 bh = __builtin_bit_cast(bf16x8, b[nb][0]);
+      const bf16x8 bl = __builtin_bit_cast(bf16x8, b[nb][1]);
       acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[nb][m], 0, 0, 0);
       acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[nb][m], 0, 0, 0);
       acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[nb][m], 0, 0, 0);
@@ -272,15 +277,15 @@ __device__ __forceinline__ void multiply(const Fragments<MT, NB, W>& f, f32x16 (
   }
 }
 
-template <int MT, int NB, int W, bool kUniformBox>
+template <int MT, int NB, int W, int kStages, bool kUniformBox>
 __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d, const LevelTable lt, int K) {
-  static_assert(kStages == 4, "the step loop below is unrolled for four ring slots");
-  constexpr int kAUnits = 2 * MT * 64, kBUnits = W * NB * 2 * 64;
+  static_assert(kStages >= 3, "the ring holds the current step and at least two in flight");
+  constexpr int kAUnits = 2 * MT * 64, kBUnits = W * NB * 2 * 64, kSlot = kAUnits + kBUnits;
   constexpr int kAPerWave = (2 * MT + W - 1) / W;
   constexpr int kDmaPerStep = kAPerWave + 2 * NB;
   static_assert((kStages - 2) * kDmaPerStep <= 63, "vmcnt holds at most 63 outstanding operations");
   // Ring slots: the class-matrix slice, then the waves' source fragments.
-  __shared__ uint4 r0[kAUnits + kBUnits], r1[kAUnits + kBUnits], r2[kAUnits + kBUnits], r3[kAUnits + kBUnits];
+  __shared__ uint4 ring[kStages * kSlot];
   __shared__ int4 offs_all[kParityOffsets];
   __shared__ int4 offs[kParityOffsets];  // the offsets some column of this workgroup uses
   __shared__ unsigned char used[W][kParityOffsets];
@@ -421,35 +426,46 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
       ioi = last ? ioi : next_oi;
       iks = last ? iks : next_ks;
     };
-    issue_step<MT, NB, W, kUniformBox>(iks, ioi, r0, r0 + kAUnits, offs, tab, d, c, sg, wave, lane);
-    advance();
-    issue_step<MT, NB, W, kUniformBox>(iks, ioi, r1, r1 + kAUnits, offs, tab, d, c, sg, wave, lane);
-    advance();
-    issue_step<MT, NB, W, kUniformBox>(iks, ioi, r2, r2 + kAUnits, offs, tab, d, c, sg, wave, lane);
-    advance();
-    // Step s: wait for this wave's DMA of step s (the two younger steps may
-    // stay in flight) and for its LDS reads of step s-1, barrier (every
-    // wave's share of slot s is in LDS; slot s-1 is free), read the
-    // fragments of slot s, refill slot s-1 with step s+3, and multiply; the
-    // products overlap the next fetch's address arithmetic.  The inline-asm
-    // barrier keeps the compiler from moving LDS accesses across it.
-#define HIPFMM_RING_STEP(S_, CUR, NXT)                                                              \
-  if ((S_) < sg.nsteps) {                                                                           \
-    __builtin_amdgcn_s_waitcnt(wait_vm_lgkm0((kStages - 2) * kDmaPerStep));                         \
-    asm volatile("s_barrier" ::: "memory");                                                         \
-    Fragments<MT, NB, W> f;                                                                         \
-    read_fragments<MT, NB, W>(r##CUR, r##CUR + kAUnits, wave, lane, f);                             \
-    issue_step<MT, NB, W, kUniformBox>(iks, ioi, r##NXT, r##NXT + kAUnits, offs, tab, d, c, sg, wave, lane); \
-    advance();                                                                                      \
-    multiply<MT, NB, W>(f, acc);                                                                    \
-  }
-    for (int s = 0; s < sg.nsteps; s += kStages) {
-      HIPFMM_RING_STEP(s, 0, 3)
-      HIPFMM_RING_STEP(s + 1, 1, 0)
-      HIPFMM_RING_STEP(s + 2, 2, 1)
-      HIPFMM_RING_STEP(s + 3, 3, 2)
+#pragma unroll
+    for (int slot = 0; slot < kStages - 1; ++slot) {
+      issue_step<MT, NB, W, kUniformBox>(iks, ioi, ring + slot * kSlot, ring + slot * kSlot + kAUnits, offs, tab, d,
+                                         c, sg, wave, lane);
+      advance();
     }
-#undef HIPFMM_RING_STEP
+    // Step s: wait for this wave's DMA of step s (the kStages - 2 younger
+    // steps may stay in flight) and for its LDS reads of step s-1, barrier
+    // (every wave's share of slot s is in LDS; slot s-1 is free), read the
+    // fragments of slot s, refill slot s-1 with step s + kStages - 1, and
+    // multiply; the products overlap the next fetch's address arithmetic.
+    // The inline-asm barrier keeps the compiler from moving LDS accesses
+    // across it.
+    //
+    // The last kHeld row tiles of each step are multiplied after the next
+    // barrier, from fragments kept in registers, so the matrix pipe has work
+    // while the next step's fragments come back from LDS (the held set
+    // starts as zero fragments: products of zeros add nothing).
+    constexpr int kHeld = MT >= 3 ? 2 : 0;
+    Fragments<MT, NB> held;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) held.a[m][0] = held.a[m][1] = uint4{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) held.b[nb][0] = held.b[nb][1] = uint4{0u, 0u, 0u, 0u};
+    int cur = 0, nxt = kStages - 1;
+    for (int s = 0; s < sg.nsteps; ++s) {
+      __builtin_amdgcn_s_waitcnt(wait_vm_lgkm0((kStages - 2) * kDmaPerStep));
+      asm volatile("s_barrier" ::: "memory");
+      Fragments<MT, NB> f;
+      read_fragments<MT, NB>(ring + cur * kSlot, ring + cur * kSlot + kAUnits, wave, lane, f);
+      issue_step<MT, NB, W, kUniformBox>(iks, ioi, ring + nxt * kSlot, ring + nxt * kSlot + kAUnits, offs, tab, d, c,
+                                         sg, wave, lane);
+      advance();
+      multiply_rows<MT, NB, MT - kHeld, MT>(held.a, held.b, acc);
+      multiply_rows<MT, NB, 0, MT - kHeld>(f.a, f.b, acc);
+      held = f;
+      cur = cur + 1 == kStages ? 0 : cur + 1;
+      nxt = nxt + 1 == kStages ? 0 : nxt + 1;
+    }
+    multiply_rows<MT, NB, MT - kHeld, MT>(held.a, held.b, acc);
     __builtin_amdgcn_s_waitcnt(0);  // the trailing repeat fetches land before the workgroup's LDS is released
   }
 
@@ -470,7 +486,7 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
   }
 }
 
-LevelTable level_table(const DevicePlan& d, int K) {
+LevelTable level_table(const DevicePlan& d, int K, int nb) {
   LevelTable lt{};
   lt.first_level = 2;
   lt.count = d.L - 1;
@@ -483,7 +499,7 @@ LevelTable level_table(const DevicePlan& d, int K) {
     const int64_t parents = int64_t(1) << (3 * (level - 1));
     const int boxes_per_tile = 32 / cpb;
     const int tiles = int((parents + boxes_per_tile - 1) / boxes_per_tile) * kchunks;
-    const int bpp = (tiles + kWaves * kNB - 1) / (kWaves * kNB);
+    const int bpp = (tiles + kWaves * nb - 1) / (kWaves * nb);
     lt.block_begin[i] = begin;
     lt.blocks_per_parity[i] = bpp;
     lt.tiles[i] = tiles;
@@ -495,15 +511,23 @@ LevelTable level_table(const DevicePlan& d, int K) {
   return lt;
 }
 
-template <int MT>
-void launch_mfma_t(const DevicePlan& d, int K, hipStream_t s) {
-  const LevelTable lt = level_table(d, K);
+template <int MT, int NB, int kStages>
+void launch_mfma_cfg(const DevicePlan& d, int K, hipStream_t s) {
+  const LevelTable lt = level_table(d, K, NB);
   const unsigned blocks = unsigned(lt.block_begin[lt.count]);
   if (K % 32 == 0)
-    m2l_mfma_kernel<MT, kNB, kWaves, true><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
+    m2l_mfma_kernel<MT, NB, kWaves, kStages, true><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
   else
-    m2l_mfma_kernel<MT, kNB, kWaves, false><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
+    m2l_mfma_kernel<MT, NB, kWaves, kStages, false><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
   HIPFMM_CHECK(hipGetLastError());
+}
+
+template <int MT>
+void launch_mfma_t(const DevicePlan& d, int K, hipStream_t s) {
+  // Two 32-column tiles per wave (192 accumulator registers) and a 4-slot
+  // ring: the largest tile that leaves registers for the fragments without
+  // spilling, and 4 * 28 KiB of LDS ring at MT = 6.
+  launch_mfma_cfg<MT, 2, 4>(d, K, s);
 }
 
 }  // namespace
