@@ -117,6 +117,8 @@ struct DevicePlan {
   int* self = nullptr;      // nt sorted self indices (-1 none)
   int2* leaf_src = nullptr; // per leaf [begin, end)
   int2* leaf_tgt = nullptr;
+  int2* p2p_tiles = nullptr; // (leaf, first row) of each 32-row near-field tile; row = 3*target + component
+  int p2p_tile_count = 0;
   // basis tables
   int4* p2m_tab = nullptr;  // per alpha: index(alpha-e_c) or -1 for c=0..2, w = sign (+1/-1)
   // operators
@@ -138,6 +140,14 @@ struct DevicePlan {
   void* M = nullptr;
   void* Lc = nullptr;
   void* Hn = nullptr;
+  // Harmonic-subspace M2L: L = Q sum_o C_o P M with the level scalings folded
+  // into P and Q (nc = 0: the M2L runs on the full n coefficients).
+  int nc = 0;
+  void* cmp_op = nullptr;  // [(level)][alpha n][NPc] = P[r][alpha] * ms[level][alpha], NPc = nc rounded up to 16
+  void* exp_op = nullptr;  // [(level)][r nc][NP]    = ls[level][beta] * Q[beta][r], NP = n rounded up to 8
+  void* unit_scale = nullptr;  // (L+1) x nc ones: the M2L's own scalings in compressed form
+  void* Mc = nullptr;      // compressed multipoles [node][nc][K] (vector-ALU M2L input)
+  void* Lcc = nullptr;     // compressed locals [node][nc][K] (M2L output)
   // MFMA operands
   void* mt_hi = nullptr;  // bf16/fp16 [node][K][apad]
   void* mt_lo = nullptr;
@@ -168,10 +178,27 @@ template <class S>
 void launch_zero(S* ptr, size_t count, hipStream_t s);
 
 void launch_m2l_split(const DevicePlan& d, int K, int mode, hipStream_t s);  // all levels 2..L
+// Harmonic-subspace transforms around the M2L, levels 2..L: M -> Mc (or the
+// MFMA split layout when `split`), and Lcc -> Lc.
+template <class S>
+void launch_m2l_compress(const DevicePlan& d, int K, bool split, hipStream_t s);
+template <class S>
+void launch_m2l_expand(const DevicePlan& d, int K, hipStream_t s);
 void launch_m2l_mfma(const DevicePlan& d, int K, int mode, hipStream_t s);   // all levels 2..L
-void prepare_mfma_operators(DevicePlan& d, const std::vector<double>& matrices, int classes,
-                            const std::vector<int>& slot_of_class, const std::vector<double>& local_scaling,
-                            std::vector<void*>& allocations, size_t& bytes);
+void prepare_mfma_operators(DevicePlan& d, int n, const std::vector<double>& matrices, int classes,
+                            const std::vector<double>& local_scaling, std::vector<void*>& allocations, size_t& bytes);
+
+// Harmonic subspace factorisation of the M2L class matrices (host, FP64):
+// T_o = Q C_o P with orthonormal block-diagonal bases; nc = 0 when the
+// matrices are not of that form.
+struct HarmonicBasis {
+  int nc = 0;              // (p+1)^2, or 0
+  std::vector<double> P;   // nc x n row-major: compressed multipole = P * multipole
+  std::vector<double> Q;   // n x nc row-major: local = Q * compressed local
+  std::vector<double> C;   // classes x nc x nc, column-major C[beta' + nc*alpha']
+  double max_reconstruction_error = 0.0;
+};
+HarmonicBasis harmonic_m2l_basis(int p, const double* T, int classes);
 
 void direct_field_fp64(const double* targets, int nt, const double* sources, int ns, const double* moments,
                        const int* self, double* fields);

@@ -42,6 +42,14 @@ __device__ inline S axis_power(S x, int g) {
   return v;
 }
 
+// bf16 bits of x, round to nearest even (inf / nan keep their class).
+__device__ inline uint16_t bf16_round(float x) {
+  uint32_t u = __builtin_bit_cast(uint32_t, x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
 // R consecutive scalars from 16-byte aligned memory (LDS or global).
 template <class S, int R>
 __device__ __forceinline__ void load_vec(const S* p, S (&out)[R]) {
@@ -440,6 +448,85 @@ __global__ void __launch_bounds__(1024) l2p_kernel(const DevicePlan d, S* __rest
   }
 }
 
+// Harmonic-subspace compression of the multipoles of one box at `level`:
+//   Mc[r][k] = sum_alpha cmp[level][alpha][r] M[alpha][k],  r < NPc,
+// written as plain scalars (Mc) or, for the MFMA M2L, split into bf16 hi/lo
+// halves in its operand layout mt[node][ks][k][16] (r = 16 ks + position;
+// rows nc..NPc-1 are zero because the operator's padding columns are).
+template <class S, bool kSplit>
+__global__ void __launch_bounds__(1024) compress_kernel(const DevicePlan d, int level, int K, int KC, int KP) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  S* B = reinterpret_cast<S*>(smem_raw);  // [n][KP]
+  const int n = d.n, NPc = round_up(d.nc, 16), KT = KP / kRK;
+  const int tid = threadIdx.x;
+  const int at = tid / KT, kt = tid % KT;
+  const bool active = at < NPc / kRA;
+  const int kc0 = blockIdx.y * KC, KCn = min(KC, K - kc0);
+  const int64_t node = level_offset(level) + blockIdx.x;
+  stage_box<S>(B, static_cast<const S*>(d.M) + node * int64_t(n) * K + kc0, n, K, KCn, KP);
+  __syncthreads();
+  if (!active) return;
+  S acc[kRA][kRK];
+#pragma unroll
+  for (int i = 0; i < kRA; ++i)
+#pragma unroll
+    for (int j = 0; j < kRK; ++j) acc[i][j] = S(0);
+  const S* A = static_cast<const S*>(d.cmp_op) + size_t(level) * n * NPc + at * kRA;
+  tile_gemm<S>(A, NPc, B + kt * kRK, KP, n, acc);
+  if constexpr (kSplit) {
+    // FP32 only: 8 consecutive rows of one 16-deep step -> one 16-byte store
+    // per state and half.
+    const int r0 = at * kRA, ks = r0 / 16, pos = r0 % 16, KS = NPc / 16;
+    uint4* hi = static_cast<uint4*>(d.mt_hi);
+    uint4* lo = static_cast<uint4*>(d.mt_lo);
+#pragma unroll
+    for (int j = 0; j < kRK; ++j) {
+      const int k = kt * kRK + j;
+      if (k >= KCn) break;
+      uint16_t h[kRA], l[kRA];
+#pragma unroll
+      for (int i = 0; i < kRA; ++i) {
+        const float x = float(acc[i][j]);
+        h[i] = bf16_round(x);
+        l[i] = bf16_round(x - __builtin_bit_cast(float, uint32_t(h[i]) << 16));
+      }
+      const int64_t at16 = (((node * KS + ks) * K + kc0 + k) * 16 + pos) / 8;  // uint4 index
+      hi[at16] = uint4{uint32_t(h[0]) | (uint32_t(h[1]) << 16), uint32_t(h[2]) | (uint32_t(h[3]) << 16),
+                       uint32_t(h[4]) | (uint32_t(h[5]) << 16), uint32_t(h[6]) | (uint32_t(h[7]) << 16)};
+      lo[at16] = uint4{uint32_t(l[0]) | (uint32_t(l[1]) << 16), uint32_t(l[2]) | (uint32_t(l[3]) << 16),
+                       uint32_t(l[4]) | (uint32_t(l[5]) << 16), uint32_t(l[6]) | (uint32_t(l[7]) << 16)};
+    }
+  } else {
+    store_tile<S, false>(static_cast<S*>(d.Mc) + node * int64_t(d.nc) * K + kc0, at * kRA, d.nc, kt * kRK, KCn, K,
+                         acc);
+  }
+}
+
+// Expansion of the compressed locals of one box at `level`:
+//   Lc[beta][k] = sum_r exp[level][r][beta] Lcc[r][k]   (overwrites Lc).
+template <class S>
+__global__ void __launch_bounds__(1024) expand_kernel(const DevicePlan d, int level, int K, int KC, int KP) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  S* B = reinterpret_cast<S*>(smem_raw);  // [nc][KP]
+  const int n = d.n, nc = d.nc, NP = round_up(n, kRA), KT = KP / kRK;
+  const int tid = threadIdx.x;
+  const int at = tid / KT, kt = tid % KT;
+  const bool active = at < NP / kRA;
+  const int kc0 = blockIdx.y * KC, KCn = min(KC, K - kc0);
+  const int64_t node = level_offset(level) + blockIdx.x;
+  stage_box<S>(B, static_cast<const S*>(d.Lcc) + node * int64_t(nc) * K + kc0, nc, K, KCn, KP);
+  __syncthreads();
+  if (!active) return;
+  S acc[kRA][kRK];
+#pragma unroll
+  for (int i = 0; i < kRA; ++i)
+#pragma unroll
+    for (int j = 0; j < kRK; ++j) acc[i][j] = S(0);
+  const S* A = static_cast<const S*>(d.exp_op) + size_t(level) * nc * NP + at * kRA;
+  tile_gemm<S>(A, NP, B + kt * kRK, KP, nc, acc);
+  store_tile<S, false>(static_cast<S*>(d.Lc) + node * int64_t(n) * K + kc0, at * kRA, n, kt * kRK, KCn, K, acc);
+}
+
 template <class S>
 __global__ void zero_kernel(S* p, size_t count) {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
@@ -517,6 +604,40 @@ void launch_l2p_unsort(const DevicePlan& d, S* fields, int K, hipStream_t s) {
 }
 
 template <class S>
+void launch_m2l_compress(const DevicePlan& d, int K, bool split, hipStream_t s) {
+  const int NPc = round_up(d.nc, 16);
+  const int kc = state_chunk(K, [&](int c) { return size_t(d.n) * round_up(c, kRK) * sizeof(S); });
+  const int KP = round_up(kc, kRK);
+  for (int level = 2; level <= d.L; ++level) {
+    const dim3 grid(unsigned(int64_t(1) << (3 * level)), unsigned((K + kc - 1) / kc));
+    const unsigned threads = tile_threads(NPc / kRA, KP);
+    const size_t lds = size_t(d.n) * KP * sizeof(S);
+    if constexpr (sizeof(S) == 4) {
+      if (split) {
+        compress_kernel<S, true><<<grid, threads, lds, s>>>(d, level, K, kc, KP);
+        HIPFMM_CHECK(hipGetLastError());
+        continue;
+      }
+    }
+    if (split) throw invalid_argument("the split compressed layout is an FP32 operand");
+    compress_kernel<S, false><<<grid, threads, lds, s>>>(d, level, K, kc, KP);
+    HIPFMM_CHECK(hipGetLastError());
+  }
+}
+
+template <class S>
+void launch_m2l_expand(const DevicePlan& d, int K, hipStream_t s) {
+  const int NP = round_up(d.n, kRA);
+  const int kc = state_chunk(K, [&](int c) { return size_t(d.nc) * round_up(c, kRK) * sizeof(S); });
+  const int KP = round_up(kc, kRK);
+  for (int level = 2; level <= d.L; ++level) {
+    const dim3 grid(unsigned(int64_t(1) << (3 * level)), unsigned((K + kc - 1) / kc));
+    expand_kernel<S><<<grid, tile_threads(NP / kRA, KP), size_t(d.nc) * KP * sizeof(S), s>>>(d, level, K, kc, KP);
+    HIPFMM_CHECK(hipGetLastError());
+  }
+}
+
+template <class S>
 void launch_zero(S* ptr, size_t count, hipStream_t s) {
   if (count == 0) return;
   const unsigned blocks = unsigned(std::min<size_t>((count + kThreads - 1) / kThreads, 65536));
@@ -530,6 +651,8 @@ void launch_zero(S* ptr, size_t count, hipStream_t s) {
   template void launch_m2m<S>(const DevicePlan&, int, int, hipStream_t);           \
   template void launch_l2l<S>(const DevicePlan&, int, int, hipStream_t);           \
   template void launch_l2p_unsort<S>(const DevicePlan&, S*, int, hipStream_t);     \
+  template void launch_m2l_compress<S>(const DevicePlan&, int, bool, hipStream_t);  \
+  template void launch_m2l_expand<S>(const DevicePlan&, int, hipStream_t);         \
   template void launch_zero<S>(S*, size_t, hipStream_t);
 HIPFMM_INSTANTIATE(float)
 HIPFMM_INSTANTIATE(double)

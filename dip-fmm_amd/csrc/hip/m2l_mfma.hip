@@ -557,11 +557,8 @@ void launch_m2l_mfma(const DevicePlan& d, int K, int mode, hipStream_t s) {
   }
 }
 
-void prepare_mfma_operators(DevicePlan& d, const std::vector<double>& matrices, int classes,
-                            const std::vector<int>& slot_of_class, const std::vector<double>& local_scaling,
-                            std::vector<void*>& allocations, size_t& bytes) {
-  (void)slot_of_class;
-  const int n = d.n;
+void prepare_mfma_operators(DevicePlan& d, int n, const std::vector<double>& matrices, int classes,
+                            const std::vector<double>& local_scaling, std::vector<void*>& allocations, size_t& bytes) {
   d.apad = ((n + 15) / 16) * 16;
   d.bpad = ((n + 31) / 32) * 32;
   const int MT = d.bpad / 32, KS = d.apad / 16;

@@ -203,9 +203,97 @@ __global__ void __launch_bounds__(kThreads) direct_kernel(const double* __restri
 
 }  // namespace
 
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// List-1 P2P on the matrix cores (v_mfma_f32_32x32x2_f32, FP32 products and
+// accumulation), for FP32 plans with K a multiple of 32.  For 32 moment
+// states the near field of a leaf is one GEMM:
+//   H[(t,c)][k] = sum_{(s,c')} D_{cc'}(t,s) m[s][c'][k],
+// rows (target, component) packed 3 per target (row = 3*t_local + c), depth
+// (source, component), columns k.  One wave owns a 32-row tile: each lane
+// forms the pair tensor entries D_{c,0..2}(t,s) of its row's target and one
+// of the two sources of a step (A operand, lane i: row i%32, depth i/32) and
+// loads m[s][c'][k] (B operand, lane i: column i%32, depth i/32); the
+// identity pair and padding rows/sources contribute zero, coincident pairs
+// NaN as in the vector path.  Waves are independent: no LDS, no barriers.
+constexpr int kP2PWaves = 4;
+
+__global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePlan d, int K) {
+  const int wave = int(threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int item = int(blockIdx.x) * kP2PWaves + wave;
+  if (item >= d.p2p_tile_count) return;
+  const int2 tile = d.p2p_tiles[item];
+  const int leaf = tile.x;
+  const int2 trange = d.leaf_tgt[leaf];
+  const int rows = 3 * (trange.y - trange.x);
+  const int col = lane & 31, g = lane >> 5;
+  const int row = tile.y + col;  // A-operand row of this lane
+  const bool row_ok = row < rows;
+  const int t = trange.x + (row_ok ? row / 3 : 0);
+  const int c = row_ok ? row % 3 : 0;
+  const PairPos<float> tp = PairPos<float>::load(d.tgt_pos, d.nt, t);
+  const int self = row_ok ? d.self[t] : -2;
+  const int k = int(blockIdx.y) * 32 + col;  // B / C column of this lane
+  const float c4pi = 0.079577471545947667884441881686257181f;
+  const float* sm = static_cast<const float*>(d.sm);
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  const int nb = 1 << d.L;
+  int lx, ly, lz;
+  demorton3(uint32_t(leaf), d.L, lx, ly, lz);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = lx + dx, y = ly + dy, z = lz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+        const int2 sr = d.leaf_src[morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L)];
+        for (int j0 = sr.x; j0 < sr.y; j0 += 2) {
+          const int s = j0 + g;
+          const bool src_ok = s < sr.y;
+          const int sl = src_ok ? s : sr.x;
+          float rx, ry, rz;
+          tp.diff(PairPos<float>::load(d.src_pos, d.ns, sl), rx, ry, rz);
+          const float r2 = rx * rx + ry * ry + rz * rz;
+          const float rinv = inv_sqrt(r2);
+          const float f = rinv * rinv * rinv * c4pi;
+          const float ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
+          const float uc = c == 0 ? ux : (c == 1 ? uy : uz);
+          const float f3uc = 3.0f * f * uc;
+          const bool use = row_ok & src_ok & (s != self);
+          const float a0 = use ? f3uc * ux - (c == 0 ? f : 0.0f) : 0.0f;
+          const float a1 = use ? f3uc * uy - (c == 1 ? f : 0.0f) : 0.0f;
+          const float a2 = use ? f3uc * uz - (c == 2 ? f : 0.0f) : 0.0f;
+          const float* ms = sm + int64_t(sl) * 3 * K + k;
+          const float b0 = src_ok ? ms[0] : 0.0f;
+          const float b1 = src_ok ? ms[K] : 0.0f;
+          const float b2 = src_ok ? ms[2 * K] : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, acc, 0, 0, 0);
+        }
+      }
+  // acc[r]: row 8*(r/4) + 4*g + r%4 of the tile, column col.
+  float* Hn = static_cast<float*>(d.Hn);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int orow = tile.y + 8 * (r >> 2) + 4 * g + (r & 3);
+    if (orow < rows) Hn[(int64_t(trange.x) * 3 + orow) * K + k] = acc[r];
+  }
+}
+
 template <class S>
 void launch_p2p(const DevicePlan& d, int K, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
+  if constexpr (sizeof(S) == 4) {
+    if (K % 32 == 0) {
+      if (d.p2p_tile_count == 0) return;
+      const dim3 grid(unsigned((d.p2p_tile_count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
+      p2p_mfma_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, K);
+      HIPFMM_CHECK(hipGetLastError());
+      return;
+    }
+  }
   if (K <= 8)
     launch_p2p_t<S, 1>(d, K, s);
   else if (K <= 32)

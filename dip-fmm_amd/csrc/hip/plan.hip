@@ -280,6 +280,13 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   }
   d.leaf_src = upload(P, ls.data(), ls.size());
   d.leaf_tgt = upload(P, lt.data(), lt.size());
+  // Near-field row tiles of the matrix-core P2P: 32 rows of (target,
+  // component) pairs of one leaf each.
+  std::vector<int2> tiles;
+  for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
+    for (int r0 = 0; r0 < 3 * (lt[leaf].y - lt[leaf].x); r0 += 32) tiles.push_back(int2{int(leaf), r0});
+  d.p2p_tile_count = int(tiles.size());
+  d.p2p_tiles = upload(P, tiles.data(), std::max<size_t>(tiles.size(), 1));
   std::vector<int> self(std::max(d.nt, 1), -1);
   if (desc.fixed_self_indices) {
     for (int t = 0; t < d.nt; ++t) {
@@ -312,6 +319,29 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   const size_t nn = size_t(n) * n;
   std::vector<S> T(std::max<size_t>(size_t(desc.m2l_class_count) * nn, 1));
   for (size_t i = 0; i < size_t(desc.m2l_class_count) * nn; ++i) T[i] = S(desc.m2l_matrices[i]);
+  // Harmonic-subspace form of the M2L (T_o = Q C_o P): class matrices of
+  // size nc x nc and the level scalings folded into P and Q.
+  hipfmm::HarmonicBasis hb;
+  if (L >= 2 && desc.m2l_class_count > 0) hb = hipfmm::harmonic_m2l_basis(p, desc.m2l_matrices, desc.m2l_class_count);
+  if (hb.nc > 0) {
+    const int nc = hb.nc, NPc = (nc + 15) / 16 * 16, NP = (n + 7) / 8 * 8;
+    d.nc = nc;
+    T.assign(std::max<size_t>(hb.C.size(), 1), S(0));
+    for (size_t i = 0; i < hb.C.size(); ++i) T[i] = S(hb.C[i]);
+    std::vector<S> cmp(size_t(L + 1) * n * NPc, S(0)), expd(size_t(L + 1) * nc * NP, S(0));
+    std::vector<S> ones(size_t(L + 1) * nc, S(1));
+    for (int l = 0; l <= L; ++l)
+      for (int r = 0; r < nc; ++r) {
+        for (int a = 0; a < n; ++a)
+          cmp[(size_t(l) * n + a) * NPc + r] = S(hb.P[size_t(r) * n + a] * desc.m2l_multipole_scaling[size_t(l) * n + a]);
+        for (int b = 0; b < n; ++b)
+          expd[(size_t(l) * nc + r) * NP + b] = S(desc.m2l_local_scaling[size_t(l) * n + b] * hb.Q[size_t(b) * nc + r]);
+      }
+    d.cmp_op = upload(P, cmp.data(), cmp.size());
+    d.exp_op = upload(P, expd.data(), expd.size());
+    d.unit_scale = upload(P, ones.data(), ones.size());
+    P->stats.m2l_reconstruction_error = hb.max_reconstruction_error;
+  }
   d.m2l_T = upload(P, T.data(), T.size());
   d.m2l_slot = upload(P, slot.data(), slot.size());
   // The 189 list-2 offsets of each parity class (target coordinate parity
@@ -400,13 +430,19 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   }
   P->stats.m2l_interaction_count = m2l_pairs;
   P->stats.m2l_leaf_interaction_count = leaf_pairs;
+  P->stats.m2l_operand_size = uint64_t(d.nc > 0 ? d.nc : n);
   P->stats.p2p_interaction_count = p2p_pairs;
   if constexpr (sizeof(S) == 4) {
     if (L >= 2 && desc.m2l_class_count > 0) {
-      std::vector<double> mats(desc.m2l_matrices, desc.m2l_matrices + size_t(desc.m2l_class_count) * nn);
-      std::vector<double> lsd(desc.m2l_local_scaling, desc.m2l_local_scaling + size_t(L + 1) * n);
       size_t bytes = 0;
-      hipfmm::prepare_mfma_operators(d, mats, desc.m2l_class_count, slot_of_class, lsd, P->persistent, bytes);
+      if (d.nc > 0) {
+        const std::vector<double> ones(size_t(L + 1) * d.nc, 1.0);
+        hipfmm::prepare_mfma_operators(d, d.nc, hb.C, desc.m2l_class_count, ones, P->persistent, bytes);
+      } else {
+        std::vector<double> mats(desc.m2l_matrices, desc.m2l_matrices + size_t(desc.m2l_class_count) * nn);
+        std::vector<double> lsd(desc.m2l_local_scaling, desc.m2l_local_scaling + size_t(L + 1) * n);
+        hipfmm::prepare_mfma_operators(d, n, mats, desc.m2l_class_count, lsd, P->persistent, bytes);
+      }
       P->persistent_bytes += bytes;
       P->stats.setup_h2d_bytes += bytes;
       P->mfma_ready = true;
@@ -439,6 +475,13 @@ void ensure_capacity(hipfmm_plan* P, int K) {
   d.Lc = alloc(cs, sizeof(S));
   HIPFMM_CHECK(hipMemset(d.M, 0, cs * sizeof(S)));
   HIPFMM_CHECK(hipMemset(d.Lc, 0, cs * sizeof(S)));
+  if (d.nc > 0) {
+    const size_t cc = size_t(d.nodes) * d.nc * cap;
+    d.Mc = alloc(cc, sizeof(S));
+    d.Lcc = alloc(cc, sizeof(S));
+    HIPFMM_CHECK(hipMemset(d.Mc, 0, cc * sizeof(S)));
+    HIPFMM_CHECK(hipMemset(d.Lcc, 0, cc * sizeof(S)));
+  }
   if (sizeof(S) == 4 && P->mfma_ready) {
     const size_t ms = size_t(d.nodes) * cap * d.apad;
     d.mt_hi = alloc(ms, 2);
@@ -558,13 +601,26 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
   hipfmm::launch_zero<S>(static_cast<S*>(d.Lc), d.L >= 2 ? top : size_t(d.nodes) * d.n * K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvM2M], s));
   const bool mfma = sizeof(S) == 4 && P->m2l_mode != HIPFMM_M2L_VALU && P->mfma_ready;
-  if (mfma) hipfmm::launch_m2l_split(d, K, P->m2l_mode, s);
+  // The M2L kernels see the plan through `dm`: in harmonic-subspace form the
+  // operands are the compressed multipoles/locals with unit scalings.
+  DevicePlan dm = d;
+  if (d.nc > 0 && d.L >= 2) {
+    dm.n = d.nc;
+    dm.M = d.Mc;
+    dm.Lc = d.Lcc;
+    dm.ms = d.unit_scale;
+    dm.ls = d.unit_scale;
+    hipfmm::launch_m2l_compress<S>(d, K, mfma, s);
+  } else if (mfma) {
+    hipfmm::launch_m2l_split(d, K, P->m2l_mode, s);
+  }
   HIPFMM_CHECK(hipEventRecord(e[kEvM2LKernel0], s));
   if (mfma)
-    hipfmm::launch_m2l_mfma(d, K, P->m2l_mode, s);
+    hipfmm::launch_m2l_mfma(dm, K, P->m2l_mode, s);
   else
-    hipfmm::launch_m2l_valu<S>(d, K, s);
+    hipfmm::launch_m2l_valu<S>(dm, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvM2LKernel1], s));
+  if (d.nc > 0 && d.L >= 2) hipfmm::launch_m2l_expand<S>(d, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvM2L], s));
   for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(d, level, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvL2L], s));

@@ -595,6 +595,8 @@ HipPlanStatistics UniformFmm::hip_plan_statistics() const {
   o.static_upload_count = s.static_upload_count;
   o.batch_capacity = s.batch_capacity;
   o.m2l_leaf_interaction_count = s.m2l_leaf_interaction_count;
+  o.m2l_operand_size = s.m2l_operand_size;
+  o.m2l_reconstruction_error = s.m2l_reconstruction_error;
   o.setup_seconds = setup_seconds_;
   return o;
 }

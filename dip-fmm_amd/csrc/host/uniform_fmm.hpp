@@ -128,7 +128,9 @@ struct HipPlanStatistics {
       p2p_interaction_count{0}, setup_h2d_bytes{0}, evaluation_h2d_bytes{0}, evaluation_d2h_bytes{0},
       evaluation_h2d_calls{0}, evaluation_d2h_calls{0}, persistent_device_bytes{0}, plan_generation_count{0},
       static_upload_count{0}, batch_capacity{0}, m2l_leaf_interaction_count{0};
-  double setup_seconds{0.0};  // host plan construction + upload
+  std::size_t m2l_operand_size{0};       // n, or (p+1)^2 in harmonic-subspace form
+  double m2l_reconstruction_error{0.0};  // of the harmonic-subspace factorisation
+  double setup_seconds{0.0};             // host plan construction + upload
 };
 
 bool hip_full_available() noexcept;

@@ -290,6 +290,8 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
                                d["static_upload_count"] = s.static_upload_count;
                                d["batch_capacity"] = s.batch_capacity;
                                d["m2l_leaf_interaction_count"] = s.m2l_leaf_interaction_count;
+                               d["m2l_operand_size"] = s.m2l_operand_size;
+                               d["m2l_reconstruction_error"] = s.m2l_reconstruction_error;
                                d["setup_seconds"] = s.setup_seconds;
                                return d;
                              })

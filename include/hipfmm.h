@@ -126,6 +126,11 @@ typedef struct hipfmm_stats {
   uint64_t static_upload_count;
   uint64_t batch_capacity;        /* moment states the state buffers hold */
   uint64_t m2l_leaf_interaction_count; /* non-empty list-2 pairs at the leaf level */
+  /* Coefficients per box the M2L product runs on: n = (p+1)(p+2)(p+3)/6, or
+   * (p+1)^2 when the class matrices factor through the harmonic subspace
+   * (T_o = Q C_o P; no counterpart in the reference, which always uses n). */
+  uint64_t m2l_operand_size;
+  double m2l_reconstruction_error; /* max |T_o - Q C_o P| / max |T_o| over classes (0 if unused) */
 } hipfmm_stats;
 
 /* Device-stream stage times of the most recent evaluate, seconds; mirrors

@@ -163,6 +163,57 @@ def test_mfma_bf16x3_within_bar_of_fp64_fmm(hip, p, depth, n):
     assert e_mfma <= 1e-5
 
 
+@pytest.mark.parametrize("p,depth,n,K,mode", [(6, 3, 12000, 32, "VALU"), (8, 3, 20000, 32, "MFMA_BF16X3"),
+                                               (4, 3, 8000, 64, "MFMA_BF16X3")])
+def test_batched_matrix_core_paths_match_fp64_fmm(hip, p, depth, n, K, mode):
+    # K a multiple of 32: the near field runs on the matrix cores (FP32
+    # products), and with the MFMA mode the M2L takes the one-box-per-tile
+    # path; every checked state must stay within the FP32 bar of the FP64 FMM.
+    rng = np.random.default_rng(400 + p + K)
+    x = rng.uniform(0, 1, size=(n, 3))
+    ms = rng.normal(size=(K, n, 3))
+    ids = np.arange(n)
+    announce("batched matrix-core", p, depth, n, K, mode)
+    f = make(hip, x, x, p, depth, 32, ids=ids, mode=getattr(hip.HipM2LMode, mode))
+    hb = f.evaluate_batch(ms)
+    ref = O.CpuFmm(x, x, p, depth, 64, fixed_ids=ids)
+    for k in (0, K // 2, K - 1):
+        e = rel_l2(hb[k], ref.evaluate(ms[k])["H"])
+        print(f"  state {k}: rel L2 vs FP64 FMM {e:.3e}", flush=True)
+        assert e <= 1e-5
+
+
+def test_batched_near_field_nan_and_self_contract(hip):
+    # The matrix-core near field (K = 32) keeps the vector path's contract:
+    # the identity pair is skipped and a coincident non-identity pair is NaN.
+    x = np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3], [0.8, 0.7, 0.6], [0.4, 0.5, 0.9]])
+    ms = np.random.default_rng(9).normal(size=(32, 4, 3))
+    announce("coincident batched")
+    f = make(hip, x, x, 2, 1, 32)
+    h = f.evaluate_batch(ms, target_source_indices=[0, -1, 2, 3])
+    assert np.isnan(h[:, 0]).all() and np.isnan(h[:, 1]).all()
+    assert np.isfinite(h[:, 2]).all() and np.isfinite(h[:, 3]).all()
+    g = make(hip, x[[0, 2, 3]], x[[0, 2, 3]], 2, 1, 32, ids=[0, 1, 2])
+    hb = g.evaluate_batch(ms[:, [0, 2, 3]])
+    for k in (0, 31):
+        assert rel_l2(hb[k], g.evaluate(ms[k][[0, 2, 3]])["H"]) <= 2e-6
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_m2l_runs_in_the_harmonic_subspace(hip, prec):
+    # The class matrices factor as T_o = Q C_o P through the (p+1)^2-
+    # dimensional harmonic subspace (checked on every class at setup); the
+    # M2L then multiplies (p+1)^2-sized operands.  Depth < 2 has no M2L.
+    rng = np.random.default_rng(14)
+    x = rng.uniform(0, 1, size=(3000, 3))
+    for p in (3, 6, 8):
+        st = make(hip, x, x, p, 3, prec).hip_plan_statistics
+        assert st["m2l_operand_size"] == (p + 1) ** 2
+        assert 0.0 <= st["m2l_reconstruction_error"] <= 1e-12
+    st = make(hip, x, x, 4, 1, prec).hip_plan_statistics
+    assert st["m2l_operand_size"] == (4 + 1) * (4 + 2) * (4 + 3) // 6
+
+
 def test_identity_map_contract(hip):
     rng = np.random.default_rng(2)
     x = rng.uniform(0, 1, size=(500, 3))
