@@ -38,6 +38,9 @@
 //   mt_hi/lo   [node][depth step][state][16] bf16 -- a B fragment (column =
 //              lane%32 -> state, depth 8*(i/32)..+7) is one 16-byte load and
 //              the 32 columns of a step form one contiguous 1 KiB block
+#include <cstdlib>
+#include <string>
+
 #include "common.hpp"
 
 namespace hipfmm {
@@ -522,12 +525,37 @@ void launch_mfma_cfg(const DevicePlan& d, int K, hipStream_t s) {
   HIPFMM_CHECK(hipGetLastError());
 }
 
+// Tile shape per row-tile count MT: NB 32-column tiles per wave (NB * MT *
+// 16 accumulator registers, at most 192 so the fragments fit beside them) and
+// the ring depth that fits LDS (160 KiB per workgroup).  The shape only
+// regroups whole output columns, so every shape gives bitwise the same
+// locals.  HIPFMM_M2L_MFMA_SHAPE=NBxS (2x4, 3x4, 4x3, 2x6) overrides the choice among
+// the compiled shapes, for measurement.
+inline int mfma_shape_override() {
+  static const int v = [] {
+    const char* e = std::getenv("HIPFMM_M2L_MFMA_SHAPE");
+    if (!e) return 0;
+    if (std::string(e) == "2x4") return 24;
+    if (std::string(e) == "3x4") return 34;
+    if (std::string(e) == "4x3") return 43;
+    if (std::string(e) == "2x6") return 26;
+    return 0;
+  }();
+  return v;
+}
+
 template <int MT>
 void launch_mfma_t(const DevicePlan& d, int K, hipStream_t s) {
-  // Two 32-column tiles per wave (192 accumulator registers) and a 4-slot
-  // ring: the largest tile that leaves registers for the fragments without
-  // spilling, and 4 * 28 KiB of LDS ring at MT = 6.
-  launch_mfma_cfg<MT, 2, 4>(d, K, s);
+  if constexpr (MT <= 3) {
+    switch (mfma_shape_override()) {
+      case 24: launch_mfma_cfg<MT, 2, 4>(d, K, s); return;
+      case 34: launch_mfma_cfg<MT, 3, 4>(d, K, s); return;
+      case 26: launch_mfma_cfg<MT, 2, 6>(d, K, s); return;
+      default: launch_mfma_cfg<MT, 4, 3>(d, K, s); return;
+    }
+  } else {
+    launch_mfma_cfg<MT, 2, 4>(d, K, s);
+  }
 }
 
 }  // namespace

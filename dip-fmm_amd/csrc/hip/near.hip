@@ -248,9 +248,18 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePl
         const int x = lx + dx, y = ly + dy, z = lz + dz;
         if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
         const int2 sr = d.leaf_src[morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L)];
-        for (int j0 = sr.x; j0 < sr.y; j0 += 2) {
+        // Blocked summation: every block of up to 32 sources accumulates into
+        // a fresh chain, added to the total once (fewer additions into one
+        // large sum).
+        for (int jb = sr.x; jb < sr.y; jb += 32) {
+        const int je = min(sr.y, jb + 32);
+        f32x16 part;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) part[r] = 0.0f;
+#pragma unroll 4
+        for (int j0 = jb; j0 < je; j0 += 2) {
           const int s = j0 + g;
-          const bool src_ok = s < sr.y;
+          const bool src_ok = s < je;
           const int sl = src_ok ? s : sr.x;
           float rx, ry, rz;
           tp.diff(PairPos<float>::load(d.src_pos, d.ns, sl), rx, ry, rz);
@@ -268,9 +277,12 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePl
           const float b0 = src_ok ? ms[0] : 0.0f;
           const float b1 = src_ok ? ms[K] : 0.0f;
           const float b2 = src_ok ? ms[2 * K] : 0.0f;
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, acc, 0, 0, 0);
+          part = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, part, 0, 0, 0);
+          part = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, part, 0, 0, 0);
+          part = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, part, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] += part[r];
         }
       }
   // acc[r]: row 8*(r/4) + 4*g + r%4 of the tile, column col.

@@ -102,6 +102,26 @@ def test_shallow_trees_are_pure_near_field(hip, depth):
     assert rel_l2(h, d) <= 1e-13
 
 
+def test_fp32_near_field_paths_against_direct_sum(hip):
+    # Depth 1 with a dense cloud: every pair is list-1.  K = 32 takes the
+    # matrix-core near field (FP32 MFMA), K = 1 the vector one; both are
+    # FP32 arithmetic and must sit at FP32 rounding level of the FP64 sum.
+    rng = np.random.default_rng(77)
+    n, K = 3000, 32
+    x = rng.uniform(0, 1, size=(n, 3))
+    ms = rng.normal(size=(K, n, 3))
+    ids = np.arange(n)
+    announce("fp32 near field", n, K)
+    f = make(hip, x, x, 2, 1, 32, ids=ids)
+    hb = f.evaluate_batch(ms)
+    for k in (0, 17):
+        d = O.direct(x, x, ms[k], ids)["H"]
+        e_mfma = rel_l2(hb[k], d)
+        e_valu = rel_l2(f.evaluate(ms[k])["H"], d)
+        print(f"  state {k}: rel L2 vs FP64 direct: matrix-core {e_mfma:.3e}  vector {e_valu:.3e}", flush=True)
+        assert e_mfma <= 2e-6 and e_valu <= 2e-6
+
+
 def test_expansions_match_oracle(hip):
     rng = np.random.default_rng(5)
     x = rng.uniform(0, 1, size=(4000, 3))
