@@ -151,6 +151,7 @@ struct DevicePlan {
   // MFMA operands
   void* mt_hi = nullptr;  // bf16/fp16 [node][K][apad]
   void* mt_lo = nullptr;
+  int64_t mt_zero_offset = 0;  // byte offset of the 1 KiB zero block that follows each of mt_hi / mt_lo
   void* T_hi = nullptr;   // [class][bpad][apad] 16-bit, MFMA fragment order
   void* T_lo = nullptr;
   void* ls_pad = nullptr; // (L+1) x bpad float local scaling (0 beyond n)

@@ -87,6 +87,16 @@ __device__ __forceinline__ void lds_dma16_sbase(const void* base, uint32_t lane_
                "s"(lds_address(lds_wave_base))
                : "memory", "m0");
 }
+// Array base (a kernel argument: the same on every lane) plus a per-lane
+// 32-bit byte offset.
+__device__ __forceinline__ void lds_dma16_voff(const void* base, uint32_t lane_bytes, const uint4* lds_wave_base) {
+  const uint64_t b = uint64_t(uintptr_t(base));
+  const uint64_t sb = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b))))) |
+                      (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b >> 32))))) << 32);
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(lane_bytes), "s"(sb),
+               "s"(lds_address(lds_wave_base))
+               : "memory", "m0");
+}
 #pragma clang diagnostic pop
 
 __host__ __device__ inline uint16_t bf16_bits(float x) {
@@ -167,22 +177,22 @@ struct StepGeometry {
 };
 
 // Per-workgroup step tables of the uniform-box path, built once: for each
-// used offset, the byte offset of its class-matrix block and, per tile, the
-// byte offset of the source box's multipole block (-1: outside the grid).
+// used offset, the byte offset of its class-matrix block in T_hi/T_lo and,
+// per tile, the byte offset of the source box's block in mt_hi/mt_lo, or
+// kOutside for a source outside the grid.
+constexpr uint32_t kOutside = 0xffffffffu;
+
 template <int W, int NB>
 struct StepTables {
-  long long a[kParityOffsets];
-  long long b[kParityOffsets][W * NB];
+  uint32_t a[kParityOffsets];
+  uint32_t b[kParityOffsets][W * NB];
 };
 
-__device__ __forceinline__ long long uniform_i64(long long v) {
-  const uint64_t u = uint64_t(v);
-  return (long long)(uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u))))) |
-                     (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u >> 32))))) << 32));
-}
-
 // Fetch the operands of step (ks, offset oi) into one ring slot.  Every step
-// issues the same number of DMA operations per wave.
+// issues the same number of DMA operations per wave.  The uniform-box path
+// addresses every operand as a fixed array base (SGPR pair) plus a 32-bit
+// per-lane byte offset computed on the vector ALUs; sources outside the grid
+// read the zero block that follows the multipole arrays.
 template <int MT, int NB, int W, bool kUniformBox>
 __device__ __forceinline__ void issue_step(int ks, int oi, const uint4* a_slot, const uint4* b_slot,
                                            const int4* offs, const StepTables<W, NB>& tab, const DevicePlan& d,
@@ -190,26 +200,24 @@ __device__ __forceinline__ void issue_step(int ks, int oi, const uint4* a_slot, 
   constexpr int kAChunks = 2 * MT, kAPerWave = (kAChunks + W - 1) / W;
   const int g = lane >> 5, col = lane & 31;
   if constexpr (kUniformBox) {
-    const long long ta = uniform_i64(tab.a[oi]) + (long long)ks * MT * 1024;
+    const uint32_t ta = tab.a[oi] + uint32_t(ks) * (MT * 1024) + uint32_t(lane) * 16u;
 #pragma unroll
     for (int i = 0; i < kAPerWave; ++i) {
       int ch = wave + W * i;
       if (ch >= kAChunks) ch -= kAChunks;  // repeats another wave's chunk: same bytes to the same place
       const int half = ch >= MT ? 1 : 0, m = ch - half * MT;
-      const char* base = static_cast<const char*>(half ? d.T_lo : d.T_hi) + ta + m * 1024;
-      lds_dma16_sbase(base, uint32_t(lane) * 16u, a_slot + ch * 64);
+      lds_dma16_voff(half ? d.T_lo : d.T_hi, ta + uint32_t(m) * 1024u, a_slot + ch * 64);
     }
-    const long long kstep = (long long)ks * sg.K * 32;
+    const uint32_t kstep = uint32_t(ks) * uint32_t(sg.K) * 32u;
     const uint32_t lane_bytes = uint32_t(col * 2 + g) * 16u;
+    const uint32_t zero = uint32_t(d.mt_zero_offset) + lane_bytes;
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      const long long tb = uniform_i64(tab.b[oi][wave * NB + nb]);
-      const bool inside = tb >= 0;
-      const char* bh = inside ? static_cast<const char*>(d.mt_hi) + tb + kstep : static_cast<const char*>(d.zero_block);
-      const char* bl = inside ? static_cast<const char*>(d.mt_lo) + tb + kstep : static_cast<const char*>(d.zero_block);
+      const uint32_t tb = tab.b[oi][wave * NB + nb];
+      const uint32_t off = tb != kOutside ? tb + kstep + lane_bytes : zero;
       const uint4* dst = b_slot + (wave * NB + nb) * 128;
-      lds_dma16_sbase(bh, lane_bytes, dst);
-      lds_dma16_sbase(bl, lane_bytes, dst + 64);
+      lds_dma16_voff(d.mt_hi, off, dst);
+      lds_dma16_voff(d.mt_lo, off, dst + 64);
     }
   } else {
     const int4 o = offs[oi];
@@ -389,19 +397,19 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
       const int a = e / (W * NB + 1), t = e - a * (W * NB + 1);
       const int4 o = offs[a];
       if (t == W * NB) {
-        tab.a[a] = (long long)o.w * sg.KS * MT * 1024;
+        tab.a[a] = uint32_t(o.w) * uint32_t(sg.KS * MT * 1024);
         continue;
       }
       const int tile = wgp * W * NB + t;
       const int bg = tile / kchunks, kc = tile % kchunks;
-      long long off = -1;
+      uint32_t off = kOutside;
       if (tile < tiles && bg < parents) {
         int x, y, z;
         demorton3(uint32_t(bg), level - 1, x, y, z);
         const int sx = 2 * x + px - o.x, sy = 2 * y + py - o.y, sz = 2 * z + pz - o.z;
         if (sx >= 0 && sy >= 0 && sz >= 0 && sx < sg.nbox && sy < sg.nbox && sz < sg.nbox) {
-          const long long snode = sg.lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), level);
-          off = (snode * sg.KS * K + (long long)kc * 32) * 32;
+          const uint64_t snode = uint64_t(sg.lvl0 + morton3(uint32_t(sx), uint32_t(sy), uint32_t(sz), level));
+          off = uint32_t((snode * uint64_t(sg.KS) * uint64_t(K) + uint64_t(kc) * 32u) * 32u);
         }
       }
       tab.b[a][t] = off;
@@ -442,17 +450,6 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
     // multiply; the products overlap the next fetch's address arithmetic.
     // The inline-asm barrier keeps the compiler from moving LDS accesses
     // across it.
-    //
-    // The last kHeld row tiles of each step are multiplied after the next
-    // barrier, from fragments kept in registers, so the matrix pipe has work
-    // while the next step's fragments come back from LDS (the held set
-    // starts as zero fragments: products of zeros add nothing).
-    constexpr int kHeld = MT >= 3 ? 2 : 0;
-    Fragments<MT, NB> held;
-#pragma unroll
-    for (int m = 0; m < MT; ++m) held.a[m][0] = held.a[m][1] = uint4{0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) held.b[nb][0] = held.b[nb][1] = uint4{0u, 0u, 0u, 0u};
     int cur = 0, nxt = kStages - 1;
     for (int s = 0; s < sg.nsteps; ++s) {
       __builtin_amdgcn_s_waitcnt(wait_vm_lgkm0((kStages - 2) * kDmaPerStep));
@@ -462,13 +459,10 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
       issue_step<MT, NB, W, kUniformBox>(iks, ioi, ring + nxt * kSlot, ring + nxt * kSlot + kAUnits, offs, tab, d, c,
                                          sg, wave, lane);
       advance();
-      multiply_rows<MT, NB, MT - kHeld, MT>(held.a, held.b, acc);
-      multiply_rows<MT, NB, 0, MT - kHeld>(f.a, f.b, acc);
-      held = f;
+      multiply_rows<MT, NB, 0, MT>(f.a, f.b, acc);
       cur = cur + 1 == kStages ? 0 : cur + 1;
       nxt = nxt + 1 == kStages ? 0 : nxt + 1;
     }
-    multiply_rows<MT, NB, MT - kHeld, MT>(held.a, held.b, acc);
     __builtin_amdgcn_s_waitcnt(0);  // the trailing repeat fetches land before the workgroup's LDS is released
   }
 
@@ -518,7 +512,9 @@ template <int MT, int NB, int kStages>
 void launch_mfma_cfg(const DevicePlan& d, int K, hipStream_t s) {
   const LevelTable lt = level_table(d, K, NB);
   const unsigned blocks = unsigned(lt.block_begin[lt.count]);
-  if (K % 32 == 0)
+  // One box per 32-column tile, and every operand byte offset within 32 bits.
+  const bool uniform = K % 32 == 0 && d.mt_zero_offset + 1024 <= int64_t(0xffffffffu);
+  if (uniform)
     m2l_mfma_kernel<MT, NB, kWaves, kStages, true><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
   else
     m2l_mfma_kernel<MT, NB, kWaves, kStages, false><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);

@@ -484,8 +484,13 @@ void ensure_capacity(hipfmm_plan* P, int K) {
   }
   if (sizeof(S) == 4 && P->mfma_ready) {
     const size_t ms = size_t(d.nodes) * cap * d.apad;
-    d.mt_hi = alloc(ms, 2);
-    d.mt_lo = alloc(ms, 2);
+    // Each half is followed by a 1 KiB zero block (the operand of sources
+    // outside the grid); zeroed once here.
+    d.mt_zero_offset = int64_t(ms) * 2;
+    d.mt_hi = alloc(ms + 512, 2);
+    d.mt_lo = alloc(ms + 512, 2);
+    HIPFMM_CHECK(hipMemset(d.mt_hi, 0, (ms + 512) * 2));
+    HIPFMM_CHECK(hipMemset(d.mt_lo, 0, (ms + 512) * 2));
   }
   d.kcap = cap;
   P->stats.batch_capacity = uint64_t(cap);
