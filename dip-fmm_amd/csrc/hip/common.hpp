@@ -117,7 +117,7 @@ struct DevicePlan {
   int* self = nullptr;      // nt sorted self indices (-1 none)
   int2* leaf_src = nullptr; // per leaf [begin, end)
   int2* leaf_tgt = nullptr;
-  int2* p2p_tiles = nullptr; // (leaf, first row) of each 32-row near-field tile; row = 3*target + component
+  int2* p2p_tiles = nullptr; // (leaf, first row) of each 128-row near-field item; row = 3*target + component
   int p2p_tile_count = 0;
   // basis tables
   int4* p2m_tab = nullptr;  // per alpha: index(alpha-e_c) or -1 for c=0..2, w = sign (+1/-1)
@@ -155,7 +155,17 @@ struct DevicePlan {
   void* T_hi = nullptr;   // [class][bpad][apad] 16-bit, MFMA fragment order
   void* T_lo = nullptr;
   void* ls_pad = nullptr; // (L+1) x bpad float local scaling (0 beyond n)
-  float* lvl_scale = nullptr; // per level operand scale for FP16 mode
+  // FP16 mode (harmonic form only).  The class matrices enter as
+  // rho_j C_o sigma_k (j, k the degrees of row and column, rho and sigma
+  // powers of two that centre every degree block high in FP16's range), split
+  // into fp16 hi/lo; the compressed multipoles as 2^e_l Mc / sigma_k, with e_l
+  // from the level's largest value in the current evaluate; the epilogue
+  // multiplies by 2^-e_l / rho_j.
+  void* T16_hi = nullptr;
+  void* T16_lo = nullptr;
+  void* ls16_pad = nullptr;          // (L+1) x bpad float: 1 / rho of each row
+  void* col16_scale = nullptr;       // apad float: 1 / sigma of each column (0 beyond nc)
+  unsigned* lvl_max_bits = nullptr;  // L+1: largest |Mc / sigma| per level (float bits)
   const void* zero_block = nullptr;  // 1 KiB of zeros: the operand of sources outside the grid
   int apad = 0, bpad = 0;
 };
@@ -188,6 +198,11 @@ void launch_m2l_expand(const DevicePlan& d, int K, hipStream_t s);
 void launch_m2l_mfma(const DevicePlan& d, int K, int mode, hipStream_t s);   // all levels 2..L
 void prepare_mfma_operators(DevicePlan& d, int n, const std::vector<double>& matrices, int classes,
                             const std::vector<double>& local_scaling, std::vector<void*>& allocations, size_t& bytes);
+// FP16 operators of the harmonic form (n = nc); call after prepare_mfma_operators.
+void prepare_mfma_fp16_operators(DevicePlan& d, int n, const std::vector<double>& matrices, int classes, int order,
+                                 std::vector<void*>& allocations, size_t& bytes);
+// FP16 mode operands: per-level maxima of Mc, then the scaled fp16 split.
+void launch_m2l_fp16_split(const DevicePlan& d, int K, hipStream_t s);
 
 // Harmonic subspace factorisation of the M2L class matrices (host, FP64):
 // T_o = Q C_o P with orthonormal block-diagonal bases; nc = 0 when the

@@ -38,6 +38,7 @@
 //   mt_hi/lo   [node][depth step][state][16] bf16 -- a B fragment (column =
 //              lane%32 -> state, depth 8*(i/32)..+7) is one 16-byte load and
 //              the 32 columns of a step form one contiguous 1 KiB block
+#include <cmath>
 #include <cstdlib>
 #include <string>
 
@@ -48,6 +49,7 @@ namespace hipfmm {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kWaves = 4;  // waves per workgroup (one per SIMD)
@@ -106,6 +108,17 @@ __host__ __device__ inline uint16_t bf16_bits(float x) {
   return uint16_t(u >> 16);
 }
 __host__ __device__ inline float bf16_value(uint16_t h) { return __builtin_bit_cast(float, uint32_t(h) << 16); }
+
+// Power-of-two exponent e of a level's FP16 operand scale: the level's
+// largest |Mc / sigma| (float bits) times 2^e lies in [2^14, 2^15), the top
+// binade below FP16's largest finite value, so that the lo half of the split
+// of every value above 2^-3 is a normal FP16 number; 0 for an empty or
+// non-finite maximum.
+__host__ __device__ inline int fp16_level_exponent(unsigned max_bits) {
+  const int biased = int((max_bits >> 23) & 0xffu);
+  if (max_bits == 0u || biased == 0 || biased == 255) return 0;
+  return 15 - (biased - 126);
+}
 
 __device__ inline int level_of_node(int64_t node) {
   int level = 0;
@@ -269,26 +282,33 @@ __device__ __forceinline__ void read_fragments(const uint4* a_slot, const uint4*
   }
 }
 
-// The 3 * NB products of row tiles [M0, M1) (hi*hi + hi*lo + lo*hi).
-template <int MT, int NB, int M0, int M1>
+// The 3 * NB products of row tiles [M0, M1) (hi*hi + hi*lo + lo*hi), on
+// bf16 or (kF16) fp16 operands.
+template <bool kF16>
+__device__ __forceinline__ f32x16 mfma_split(uint4 a, uint4 b, f32x16 c) {
+  if constexpr (kF16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8, a), __builtin_bit_cast(half8, b), c, 0, 0,
+                                                  0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+
+template <int MT, int NB, int M0, int M1, bool kF16>
 __device__ __forceinline__ void multiply_rows(const uint4 (&a)[MT][2], const uint4 (&b)[NB][2],
                                               f32x16 (&acc)[NB][MT]) {
 #pragma unroll
   for (int m = M0; m < M1; ++m) {
-    const bf16x8 ah = __builtin_bit_cast(bf16x8, a[m][0]);
-    const bf16x8 al = __builtin_bit_cast(bf16x8, a[m][1]);
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      const bf16x8 bh = __builtin_bit_cast(bf16x8, b[nb][0]);
-      const bf16x8 bl = __builtin_bit_cast(bf16x8, b[nb][1]);
-      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[nb][m], 0, 0, 0);
-      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[nb][m], 0, 0, 0);
-      acc[nb][m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[nb][m], 0, 0, 0);
+      acc[nb][m] = mfma_split<kF16>(a[m][0], b[nb][0], acc[nb][m]);
+      acc[nb][m] = mfma_split<kF16>(a[m][0], b[nb][1], acc[nb][m]);
+      acc[nb][m] = mfma_split<kF16>(a[m][1], b[nb][0], acc[nb][m]);
     }
   }
 }
 
-template <int MT, int NB, int W, int kStages, bool kUniformBox>
+template <int MT, int NB, int W, int kStages, bool kUniformBox, bool kF16>
 __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d, const LevelTable lt, int K) {
   static_assert(kStages >= 3, "the ring holds the current step and at least two in flight");
   constexpr int kAUnits = 2 * MT * 64, kBUnits = W * NB * 2 * 64, kSlot = kAUnits + kBUnits;
@@ -459,14 +479,17 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
       issue_step<MT, NB, W, kUniformBox>(iks, ioi, ring + nxt * kSlot, ring + nxt * kSlot + kAUnits, offs, tab, d, c,
                                          sg, wave, lane);
       advance();
-      multiply_rows<MT, NB, 0, MT>(f.a, f.b, acc);
+      multiply_rows<MT, NB, 0, MT, kF16>(f.a, f.b, acc);
       cur = cur + 1 == kStages ? 0 : cur + 1;
       nxt = nxt + 1 == kStages ? 0 : nxt + 1;
     }
     __builtin_amdgcn_s_waitcnt(0);  // the trailing repeat fetches land before the workgroup's LDS is released
   }
 
-  const float* lsp = static_cast<const float*>(d.ls_pad) + int64_t(level) * BP;
+  const float* lsp = static_cast<const float*>(kF16 ? d.ls16_pad : d.ls_pad) + int64_t(level) * BP;
+  // FP16 operands of this level carry the power-of-two scale 2^e (see
+  // fp16_level_exponent); the products are divided by it here.
+  const float unscale = kF16 ? ldexpf(1.0f, -fp16_level_exponent(d.lvl_max_bits[level])) : 1.0f;
   float* Lc = static_cast<float*>(d.Lc);
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
@@ -478,7 +501,7 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = m * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-        if (row < n) out[int64_t(row) * K] = lsp[row] * acc[nb][m][r];
+        if (row < n) out[int64_t(row) * K] = (lsp[row] * unscale) * acc[nb][m][r];
       }
   }
 }
@@ -509,15 +532,24 @@ LevelTable level_table(const DevicePlan& d, int K, int nb) {
 }
 
 template <int MT, int NB, int kStages>
-void launch_mfma_cfg(const DevicePlan& d, int K, hipStream_t s) {
+void launch_mfma_cfg(const DevicePlan& d, int K, bool f16, hipStream_t s) {
   const LevelTable lt = level_table(d, K, NB);
   const unsigned blocks = unsigned(lt.block_begin[lt.count]);
   // One box per 32-column tile, and every operand byte offset within 32 bits.
   const bool uniform = K % 32 == 0 && d.mt_zero_offset + 1024 <= int64_t(0xffffffffu);
-  if (uniform)
-    m2l_mfma_kernel<MT, NB, kWaves, kStages, true><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
+  DevicePlan v = d;
+  if (f16) {
+    v.T_hi = d.T16_hi;
+    v.T_lo = d.T16_lo;
+  }
+  if (uniform && f16)
+    m2l_mfma_kernel<MT, NB, kWaves, kStages, true, true><<<blocks, kWaves * 64, 0, s>>>(v, lt, K);
+  else if (uniform)
+    m2l_mfma_kernel<MT, NB, kWaves, kStages, true, false><<<blocks, kWaves * 64, 0, s>>>(v, lt, K);
+  else if (f16)
+    m2l_mfma_kernel<MT, NB, kWaves, kStages, false, true><<<blocks, kWaves * 64, 0, s>>>(v, lt, K);
   else
-    m2l_mfma_kernel<MT, NB, kWaves, kStages, false><<<blocks, kWaves * 64, 0, s>>>(d, lt, K);
+    m2l_mfma_kernel<MT, NB, kWaves, kStages, false, false><<<blocks, kWaves * 64, 0, s>>>(v, lt, K);
   HIPFMM_CHECK(hipGetLastError());
 }
 
@@ -541,16 +573,16 @@ inline int mfma_shape_override() {
 }
 
 template <int MT>
-void launch_mfma_t(const DevicePlan& d, int K, hipStream_t s) {
+void launch_mfma_t(const DevicePlan& d, int K, bool f16, hipStream_t s) {
   if constexpr (MT <= 3) {
     switch (mfma_shape_override()) {
-      case 24: launch_mfma_cfg<MT, 2, 4>(d, K, s); return;
-      case 34: launch_mfma_cfg<MT, 3, 4>(d, K, s); return;
-      case 26: launch_mfma_cfg<MT, 2, 6>(d, K, s); return;
-      default: launch_mfma_cfg<MT, 4, 3>(d, K, s); return;
+      case 24: launch_mfma_cfg<MT, 2, 4>(d, K, f16, s); return;
+      case 34: launch_mfma_cfg<MT, 3, 4>(d, K, f16, s); return;
+      case 26: launch_mfma_cfg<MT, 2, 6>(d, K, f16, s); return;
+      default: launch_mfma_cfg<MT, 4, 3>(d, K, f16, s); return;
     }
   } else {
-    launch_mfma_cfg<MT, 2, 4>(d, K, s);
+    launch_mfma_cfg<MT, 2, 4>(d, K, f16, s);
   }
 }
 
@@ -568,17 +600,162 @@ void launch_m2l_split(const DevicePlan& d, int K, int mode, hipStream_t s) {
 }
 
 void launch_m2l_mfma(const DevicePlan& d, int K, int mode, hipStream_t s) {
-  (void)mode;
   if (d.L < 2) return;
+  const bool f16 = mode == 2;  // HIPFMM_M2L_MFMA_FP16X3
+  if (f16 && (d.T16_hi == nullptr || d.nc == 0))
+    throw invalid_argument("the FP16 M2L mode needs class matrices of harmonic form");
   switch (d.bpad / 32) {
-    case 1: launch_mfma_t<1>(d, K, s); break;
-    case 2: launch_mfma_t<2>(d, K, s); break;
-    case 3: launch_mfma_t<3>(d, K, s); break;
-    case 4: launch_mfma_t<4>(d, K, s); break;
-    case 5: launch_mfma_t<5>(d, K, s); break;
-    case 6: launch_mfma_t<6>(d, K, s); break;
-    default: throw invalid_argument("expansion order too large for the MFMA M2L path (p <= 8)");
+    case 1: launch_mfma_t<1>(d, K, f16, s); break;
+    case 2: launch_mfma_t<2>(d, K, f16, s); break;
+    case 3: launch_mfma_t<3>(d, K, f16, s); break;
+    case 4: launch_mfma_t<4>(d, K, f16, s); break;
+    case 5: launch_mfma_t<5>(d, K, f16, s); break;
+    case 6: launch_mfma_t<6>(d, K, f16, s); break;
+    default: throw invalid_argument("expansion order too large for the MFMA M2L path");
   }
+}
+
+namespace {
+
+// Largest |Mc / sigma| of each level (positive float bits compare as
+// unsigned; vector atomics).  One workgroup per box at levels >= 2.
+__global__ void __launch_bounds__(256) level_max_kernel(const DevicePlan d, int K) {
+  __shared__ unsigned red[256];
+  const int64_t node = level_offset(2) + blockIdx.x;
+  const int level = level_of_node(node);
+  const float* Mc = static_cast<const float*>(d.Mc) + node * int64_t(d.nc) * K;
+  const float* cs = static_cast<const float*>(d.col16_scale);
+  unsigned mx = 0u;
+  for (int i = threadIdx.x; i < d.nc * K; i += blockDim.x)
+    mx = max(mx, __builtin_bit_cast(unsigned, fabsf(Mc[i] * cs[i / K])));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(d.lvl_max_bits + level, red[0]);
+}
+
+// FP16 hi/lo split of 2^e_l Mc / sigma into the MFMA B layout
+// mt[node][ks][K][16]; each thread packs 8 consecutive coefficients of one
+// column (reads coalesced over the K columns, 16-byte stores).
+__global__ void __launch_bounds__(256) split_f16_kernel(const DevicePlan d, int K) {
+  const int nc = d.nc, AP = d.apad, KS = AP / 16;
+  const int64_t node = level_offset(2) + blockIdx.x;
+  const int level = level_of_node(node);
+  const float scale = ldexpf(1.0f, fp16_level_exponent(d.lvl_max_bits[level]));
+  const float* Mc = static_cast<const float*>(d.Mc) + node * int64_t(nc) * K;
+  const float* cs = static_cast<const float*>(d.col16_scale);
+  for (int w = threadIdx.x; w < K * (AP / 8); w += blockDim.x) {
+    const int k = w % K, a0 = (w / K) * 8;
+    half8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int a = a0 + e;
+      const float x = a < nc ? (scale * cs[a]) * Mc[int64_t(a) * K + k] : 0.0f;
+      const _Float16 h = _Float16(x);
+      hi[e] = h;
+      lo[e] = _Float16(x - float(h));
+    }
+    const int64_t at = ((node * KS + a0 / 16) * K + k) * 16 + a0 % 16;
+    *reinterpret_cast<uint4*>(static_cast<_Float16*>(d.mt_hi) + at) = __builtin_bit_cast(uint4, hi);
+    *reinterpret_cast<uint4*>(static_cast<_Float16*>(d.mt_lo) + at) = __builtin_bit_cast(uint4, lo);
+  }
+}
+
+}  // namespace
+
+void launch_m2l_fp16_split(const DevicePlan& d, int K, hipStream_t s) {
+  if (d.L < 2) return;
+  const unsigned boxes = unsigned(d.nodes - level_offset(2));
+  launch_zero<float>(reinterpret_cast<float*>(d.lvl_max_bits), size_t(d.L + 1), s);
+  level_max_kernel<<<boxes, 256, 0, s>>>(d, K);
+  HIPFMM_CHECK(hipGetLastError());
+  split_f16_kernel<<<boxes, 256, 0, s>>>(d, K);
+  HIPFMM_CHECK(hipGetLastError());
+}
+
+void prepare_mfma_fp16_operators(DevicePlan& d, int n, const std::vector<double>& matrices, int classes, int order,
+                                 std::vector<void*>& allocations, size_t& bytes) {
+  const int MT = d.bpad / 32, KS = d.apad / 16, D = order + 1;
+  if (n != D * D) throw invalid_argument("the FP16 M2L mode needs class matrices of harmonic form");
+  // Degree of each harmonic row/column (rows j^2 .. (j+1)^2 - 1 have degree j).
+  std::vector<int> deg(n);
+  for (int j = 0; j < D; ++j)
+    for (int r = j * j; r < (j + 1) * (j + 1); ++r) deg[r] = j;
+  // log2 of the largest entry of each degree block (j, k) over all classes.
+  std::vector<double> lt(size_t(D) * D, 0.0), tab(size_t(D) * D, 0.0);
+  for (int c = 0; c < classes; ++c)
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        double& t = tab[size_t(deg[b]) * D + deg[a]];
+        t = std::max(t, std::fabs(matrices[size_t(c) * n * n + b + size_t(n) * a]));
+      }
+  bool all_nonzero = true;
+  for (size_t i = 0; i < tab.size(); ++i) {
+    all_nonzero = all_nonzero && tab[i] > 0.0;
+    lt[i] = tab[i] > 0.0 ? std::log2(tab[i]) : 0.0;
+  }
+  // Column exponents: the column effect of the two-way least-squares fit
+  // log2 tab[j][k] ~ a_j + b_k (column mean minus grand mean), rounded; the
+  // block magnitudes of these operators depend on j + k mostly, so the fit
+  // leaves every block within a few binades of its row's largest.
+  std::vector<int> bexp(D, 0), rexp(D, 0);
+  if (all_nonzero) {
+    double grand = 0.0;
+    for (double v : lt) grand += v;
+    grand /= double(lt.size());
+    for (int k = 0; k < D; ++k) {
+      double col = 0.0;
+      for (int j = 0; j < D; ++j) col += lt[size_t(j) * D + k];
+      bexp[k] = int(std::lround(col / D - grand));
+    }
+  }
+  // Row exponents: the row's largest scaled block in [2^13, 2^14].
+  for (int j = 0; j < D; ++j) {
+    double mx = -1e300;
+    for (int k = 0; k < D; ++k)
+      if (tab[size_t(j) * D + k] > 0.0) mx = std::max(mx, lt[size_t(j) * D + k] - bexp[k]);
+    rexp[j] = mx > -1e300 ? 14 - int(std::ceil(mx)) : 0;
+  }
+  const size_t per_class = size_t(MT) * KS * 64 * 8;
+  std::vector<_Float16> hi(per_class * classes), lo(per_class * classes);
+  for (int c = 0; c < classes; ++c) {
+    const double* T = matrices.data() + size_t(c) * n * n;
+    for (int ks = 0; ks < KS; ++ks)
+      for (int m = 0; m < MT; ++m)
+        for (int lane = 0; lane < 64; ++lane)
+          for (int e = 0; e < 8; ++e) {
+            const int row = m * 32 + lane % 32;
+            const int dep = ks * 16 + 8 * (lane / 32) + e;
+            const float v = (row < n && dep < n)
+                                ? float(std::ldexp(T[row + size_t(n) * dep], rexp[deg[row]] - bexp[deg[dep]]))
+                                : 0.0f;
+            const size_t i = size_t(c) * per_class + ((size_t(ks) * MT + m) * 64 + lane) * 8 + e;
+            hi[i] = _Float16(v);
+            lo[i] = _Float16(v - float(hi[i]));
+          }
+  }
+  const int levels = d.L + 1;
+  std::vector<float> lsp(size_t(levels) * d.bpad, 0.0f), csc(d.apad, 0.0f);
+  for (int l = 0; l < levels; ++l)
+    for (int b = 0; b < n; ++b) lsp[size_t(l) * d.bpad + b] = std::ldexp(1.0f, -rexp[deg[b]]);
+  for (int a = 0; a < n; ++a) csc[a] = std::ldexp(1.0f, bexp[deg[a]]);
+  auto up = [&](const void* host, size_t count) {
+    void* ptr = nullptr;
+    HIPFMM_CHECK(hipMalloc(&ptr, std::max<size_t>(count, 16)));
+    allocations.push_back(ptr);
+    HIPFMM_CHECK(hipMemcpy(ptr, host, count, hipMemcpyHostToDevice));
+    bytes += count;
+    return ptr;
+  };
+  d.T16_hi = up(hi.data(), hi.size() * 2);
+  d.T16_lo = up(lo.data(), lo.size() * 2);
+  d.ls16_pad = up(lsp.data(), lsp.size() * 4);
+  d.col16_scale = up(csc.data(), csc.size() * 4);
+  const std::vector<unsigned> zeros(levels, 0u);
+  d.lvl_max_bits = static_cast<unsigned*>(up(zeros.data(), zeros.size() * sizeof(unsigned)));
 }
 
 void prepare_mfma_operators(DevicePlan& d, int n, const std::vector<double>& matrices, int classes,

@@ -60,6 +60,12 @@ struct hipfmm_plan {
   size_t pinned_in_bytes = 0, pinned_out_bytes = 0;
   std::vector<int> fixed_self;
   bool identity_initialised = false;
+  // Role-exchanged view used by the adjoint: near-field items over source
+  // leaves and the inverse identity map (sorted source -> sorted target).
+  int2* adj_tiles = nullptr;
+  int adj_tile_count = 0;
+  int* adj_self = nullptr;
+  bool adj_self_valid = true;  // false when two targets share one source
   double device_moment_scale = 1.0;
   int m2l_mode = HIPFMM_M2L_VALU;
   bool mfma_ready = false;
@@ -217,6 +223,22 @@ void* shift_dense_transposed(hipfmm_plan* P, const double* dense, int L, int n) 
   return upload(P, out.data(), out.size());
 }
 
+// Inverse of the fixed identity map for the adjoint (sorted source ->
+// sorted target, or -1).  A map with two targets on one source has no
+// inverse; the adjoint then refuses to run.
+void refresh_adjoint_self(hipfmm_plan* P) {
+  const DevicePlan& d = P->d;
+  std::vector<int> inv(std::max(d.ns, 1), -1);
+  P->adj_self_valid = true;
+  for (int t = 0; t < int(P->fixed_self.size()); ++t) {
+    const int s = P->fixed_self[t];
+    if (s < 0) continue;
+    if (inv[s] >= 0) P->adj_self_valid = false;
+    inv[s] = t;
+  }
+  if (P->adj_self && d.ns) HIPFMM_CHECK(hipMemcpy(P->adj_self, inv.data(), size_t(d.ns) * sizeof(int), hipMemcpyHostToDevice));
+}
+
 template <class S>
 void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   DevicePlan& d = P->d;
@@ -281,12 +303,22 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   d.leaf_src = upload(P, ls.data(), ls.size());
   d.leaf_tgt = upload(P, lt.data(), lt.size());
   // Near-field row tiles of the matrix-core P2P: 32 rows of (target,
-  // component) pairs of one leaf each.
+  // component) pairs of one leaf each; the adjoint's tiles run over the
+  // source leaves.
   std::vector<int2> tiles;
   for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
     for (int r0 = 0; r0 < 3 * (lt[leaf].y - lt[leaf].x); r0 += 32) tiles.push_back(int2{int(leaf), r0});
   d.p2p_tile_count = int(tiles.size());
   d.p2p_tiles = upload(P, tiles.data(), std::max<size_t>(tiles.size(), 1));
+  std::vector<int2> adj_tiles;
+  for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
+    for (int r0 = 0; r0 < 3 * (ls[leaf].y - ls[leaf].x); r0 += 32) adj_tiles.push_back(int2{int(leaf), r0});
+  P->adj_tile_count = int(adj_tiles.size());
+  P->adj_tiles = upload(P, adj_tiles.data(), std::max<size_t>(adj_tiles.size(), 1));
+  {
+    const std::vector<int> none(std::max(d.ns, 1), -1);
+    P->adj_self = upload(P, none.data(), none.size());
+  }
   std::vector<int> self(std::max(d.nt, 1), -1);
   if (desc.fixed_self_indices) {
     for (int t = 0; t < d.nt; ++t) {
@@ -296,6 +328,7 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
     }
     P->fixed_self.assign(self.begin(), self.begin() + d.nt);
     P->identity_initialised = true;
+    refresh_adjoint_self(P);
   }
   d.self = upload(P, self.data(), self.size());
   const auto tab = basis_tables(p);
@@ -438,6 +471,7 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
       if (d.nc > 0) {
         const std::vector<double> ones(size_t(L + 1) * d.nc, 1.0);
         hipfmm::prepare_mfma_operators(d, d.nc, hb.C, desc.m2l_class_count, ones, P->persistent, bytes);
+        hipfmm::prepare_mfma_fp16_operators(d, d.nc, hb.C, desc.m2l_class_count, p, P->persistent, bytes);
       } else {
         std::vector<double> mats(desc.m2l_matrices, desc.m2l_matrices + size_t(desc.m2l_class_count) * nn);
         std::vector<double> lsd(desc.m2l_local_scaling, desc.m2l_local_scaling + size_t(L + 1) * n);
@@ -467,10 +501,13 @@ void ensure_capacity(hipfmm_plan* P, int K) {
   auto alloc = [&](size_t count, size_t elem) {
     return device_alloc(P, count * elem, P->state, P->state_bytes);
   };
-  d.moments_in = alloc(size_t(std::max(d.ns, 1)) * 3 * cap, sizeof(S));
-  d.fields_out = alloc(size_t(std::max(d.nt, 1)) * 3 * cap, sizeof(S));
-  d.sm = alloc(size_t(std::max(d.ns, 1)) * 3 * cap, sizeof(S));
-  d.Hn = alloc(size_t(std::max(d.nt, 1)) * 3 * cap, sizeof(S));
+  // Point-sized buffers hold max(N_s, N_t) so that the adjoint (roles
+  // exchanged) runs on the same state.
+  const size_t points = size_t(std::max({d.ns, d.nt, 1}));
+  d.moments_in = alloc(points * 3 * cap, sizeof(S));
+  d.fields_out = alloc(points * 3 * cap, sizeof(S));
+  d.sm = alloc(points * 3 * cap, sizeof(S));
+  d.Hn = alloc(points * 3 * cap, sizeof(S));
   d.M = alloc(cs, sizeof(S));
   d.Lc = alloc(cs, sizeof(S));
   HIPFMM_CHECK(hipMemset(d.M, 0, cs * sizeof(S)));
@@ -525,30 +562,52 @@ void finalize_timings(hipfmm_plan* P) {
   P->timings_pending = false;
 }
 
+// The plan seen with the roles of sources and targets exchanged: the
+// adjoint runs every kernel of the forward evaluate on this view.
+DevicePlan role_exchanged(const hipfmm_plan* P) {
+  DevicePlan a = P->d;
+  std::swap(a.ns, a.nt);
+  std::swap(a.src_pos, a.tgt_pos);
+  std::swap(a.src_rel, a.tgt_rel);
+  std::swap(a.src_perm, a.tgt_perm);
+  std::swap(a.leaf_src, a.leaf_tgt);
+  a.self = P->adj_self;
+  a.p2p_tiles = P->adj_tiles;
+  a.p2p_tile_count = P->adj_tile_count;
+  return a;
+}
+
 template <class S>
 void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, const int* self, int memspace,
-                   hipStream_t user) {
-  DevicePlan& d = P->d;
-  require(K >= 1 && K <= hipfmm::kMaxBatch, "batch must be between 1 and 64 moment states");
-  require(moments != nullptr || d.ns == 0, "moments pointer is null");
-  require(fields != nullptr || d.nt == 0, "fields pointer is null");
-  require(memspace == HIPFMM_HOST || memspace == HIPFMM_DEVICE, "unknown memory space");
-  // Identity map: fixed with the plan on first use (src/cuda_fmm.cu:3470-3486).
-  if (self != nullptr) {
-    if (!P->identity_initialised) {
-      for (int t = 0; t < d.nt; ++t)
-        require(self[t] >= -1 && self[t] < d.ns, "identity map contains an invalid index");
-      P->fixed_self.assign(self, self + d.nt);
-      if (d.nt) HIPFMM_CHECK(hipMemcpy(d.self, self, size_t(d.nt) * sizeof(int), hipMemcpyHostToDevice));
-      P->stats.setup_h2d_bytes += size_t(d.nt) * sizeof(int);
-      P->identity_initialised = true;
-    } else if (!std::equal(self, self + d.nt, P->fixed_self.begin())) {
-      throw hipfmm::invalid_argument("HipFull identity map changed; rebuild the static plan");
+                   hipStream_t user, bool adjoint) {
+  {
+    const DevicePlan& f = P->d;
+    const int n_in = adjoint ? f.nt : f.ns, n_out = adjoint ? f.ns : f.nt;
+    require(K >= 1 && K <= hipfmm::kMaxBatch, "batch must be between 1 and 64 moment states");
+    require(moments != nullptr || n_in == 0, adjoint ? "cotangents pointer is null" : "moments pointer is null");
+    require(fields != nullptr || n_out == 0, adjoint ? "gradients pointer is null" : "fields pointer is null");
+    require(memspace == HIPFMM_HOST || memspace == HIPFMM_DEVICE, "unknown memory space");
+    // Identity map: fixed with the plan on first use (src/cuda_fmm.cu:3470-3486).
+    if (self != nullptr) {
+      if (!P->identity_initialised) {
+        for (int t = 0; t < f.nt; ++t)
+          require(self[t] >= -1 && self[t] < f.ns, "identity map contains an invalid index");
+        P->fixed_self.assign(self, self + f.nt);
+        if (f.nt) HIPFMM_CHECK(hipMemcpy(f.self, self, size_t(f.nt) * sizeof(int), hipMemcpyHostToDevice));
+        P->stats.setup_h2d_bytes += size_t(f.nt) * sizeof(int);
+        P->identity_initialised = true;
+        refresh_adjoint_self(P);
+      } else if (!std::equal(self, self + f.nt, P->fixed_self.begin())) {
+        throw hipfmm::invalid_argument("HipFull identity map changed; rebuild the static plan");
+      }
     }
+    require(!adjoint || P->adj_self_valid,
+            "the identity map pairs two targets with one source; the adjoint has no transposed form of it");
   }
   if (P->timings_pending) finalize_timings(P);
   (void)hipGetLastError();
   ensure_capacity<S>(P, K);
+  const DevicePlan d = adjoint ? role_exchanged(P) : P->d;
   hipStream_t s = P->s_far;
   hipEvent_t* e = P->ev;
   const S* mu = nullptr;
@@ -615,7 +674,11 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
     dm.Lc = d.Lcc;
     dm.ms = d.unit_scale;
     dm.ls = d.unit_scale;
-    hipfmm::launch_m2l_compress<S>(d, K, mfma, s);
+    // bf16 operands come straight out of the compression; FP16 operands need
+    // the level maxima of the compressed multipoles first.
+    const bool fp16 = mfma && P->m2l_mode == HIPFMM_M2L_MFMA_FP16X3;
+    hipfmm::launch_m2l_compress<S>(d, K, mfma && !fp16, s);
+    if (fp16) hipfmm::launch_m2l_fp16_split(d, K, s);
   } else if (mfma) {
     hipfmm::launch_m2l_split(d, K, P->m2l_mode, s);
   }
@@ -703,7 +766,6 @@ int hipfmm_plan_create(const hipfmm_plan_desc* desc, hipfmm_plan** plan) {
     require(desc->max_level == 0 || (desc->m2m_operators && desc->l2l_operators), "shift operators must not be null");
     require(desc->m2l_mode >= HIPFMM_M2L_VALU && desc->m2l_mode <= HIPFMM_M2L_MFMA_FP16X3, "unknown M2L mode");
     require(desc->precision == 32 || desc->m2l_mode == HIPFMM_M2L_VALU, "MFMA M2L modes need an FP32 plan");
-    require(desc->m2l_mode != HIPFMM_M2L_MFMA_FP16X3, "the FP16 split M2L mode is not available in this build");
     if (gfx950_devices() == 0) {
       g_last_error = "HipFull is unavailable: no HIP device";
       throw std::runtime_error(g_last_error);
@@ -720,6 +782,8 @@ int hipfmm_plan_create(const hipfmm_plan_desc* desc, hipfmm_plan** plan) {
         create_impl<float>(P.get(), *desc);
       else
         create_impl<double>(P.get(), *desc);
+      require(P->m2l_mode != HIPFMM_M2L_MFMA_FP16X3 || P->d.T16_hi != nullptr,
+              "the FP16 M2L mode needs M2L class matrices of harmonic form (max_level >= 2)");
     } catch (...) {
       hipfmm_plan_destroy(P.release());
       throw;
@@ -737,9 +801,22 @@ int hipfmm_plan_evaluate(hipfmm_plan* plan, const void* moments, void* fields, i
     DeviceGuard guard(plan->device);
     const hipStream_t user = static_cast<hipStream_t>(stream);
     if (plan->d.precision == 32)
-      evaluate_impl<float>(plan, moments, fields, batch, sorted_self_indices, memspace, user);
+      evaluate_impl<float>(plan, moments, fields, batch, sorted_self_indices, memspace, user, false);
     else
-      evaluate_impl<double>(plan, moments, fields, batch, sorted_self_indices, memspace, user);
+      evaluate_impl<double>(plan, moments, fields, batch, sorted_self_indices, memspace, user, false);
+  });
+}
+
+int hipfmm_plan_evaluate_adjoint(hipfmm_plan* plan, const void* cotangents, void* gradients, int batch,
+                                 int memspace, void* stream) {
+  return guarded([&] {
+    require(plan != nullptr, "plan is null");
+    DeviceGuard guard(plan->device);
+    const hipStream_t user = static_cast<hipStream_t>(stream);
+    if (plan->d.precision == 32)
+      evaluate_impl<float>(plan, cotangents, gradients, batch, nullptr, memspace, user, true);
+    else
+      evaluate_impl<double>(plan, cotangents, gradients, batch, nullptr, memspace, user, true);
   });
 }
 
@@ -748,7 +825,8 @@ int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode) {
     require(plan != nullptr, "plan is null");
     require(mode >= HIPFMM_M2L_VALU && mode <= HIPFMM_M2L_MFMA_FP16X3, "unknown M2L mode");
     require(mode == HIPFMM_M2L_VALU || plan->d.precision == 32, "MFMA M2L modes need an FP32 plan");
-    require(mode != HIPFMM_M2L_MFMA_FP16X3, "the FP16 split M2L mode is not available in this build");
+    require(mode != HIPFMM_M2L_MFMA_FP16X3 || plan->d.T16_hi != nullptr,
+            "the FP16 M2L mode needs M2L class matrices of harmonic form (max_level >= 2)");
     plan->m2l_mode = mode;
   });
 }

@@ -567,6 +567,36 @@ void UniformFmm::evaluate_device(const void* moments, void* fields, int batch, v
   check(hipfmm_plan_evaluate(plan_, moments, fields, batch, self, HIPFMM_DEVICE, stream));
 }
 
+void UniformFmm::evaluate_adjoint_batch_into(std::span<const Vec3> cotangents, int batch, void* gradients) {
+  const std::size_t nt = tree_.sorted_targets().size();
+  if (batch < 1 || cotangents.size() != nt * static_cast<std::size_t>(batch))
+    throw std::invalid_argument("batched cotangents must hold batch x target_count vectors");
+  // The transpose carries the same kernel, so the FP32 frame scales the
+  // cotangents as it scales the moments.
+  if (precision_ == StaticPrecision::Float32) {
+    std::vector<float> g(3 * cotangents.size());
+    for (std::size_t i = 0; i < cotangents.size(); ++i) {
+      g[3 * i] = static_cast<float>(cotangents[i].x / scale_ / scale_ / scale_);
+      g[3 * i + 1] = static_cast<float>(cotangents[i].y / scale_ / scale_ / scale_);
+      g[3 * i + 2] = static_cast<float>(cotangents[i].z / scale_ / scale_ / scale_);
+    }
+    check(hipfmm_plan_evaluate_adjoint(plan_, g.data(), gradients, batch, HIPFMM_HOST, nullptr));
+  } else {
+    std::vector<double> g(3 * cotangents.size());
+    for (std::size_t i = 0; i < cotangents.size(); ++i) {
+      g[3 * i] = cotangents[i].x;
+      g[3 * i + 1] = cotangents[i].y;
+      g[3 * i + 2] = cotangents[i].z;
+    }
+    check(hipfmm_plan_evaluate_adjoint(plan_, g.data(), gradients, batch, HIPFMM_HOST, nullptr));
+  }
+  record_timings();
+}
+
+void UniformFmm::evaluate_adjoint_device(const void* cotangents, void* gradients, int batch, void* stream) {
+  check(hipfmm_plan_evaluate_adjoint(plan_, cotangents, gradients, batch, HIPFMM_DEVICE, stream));
+}
+
 void UniformFmm::set_m2l_mode(HipM2LMode mode) {
   check(hipfmm_plan_set_m2l_mode(plan_, static_cast<int>(mode)));
   m2l_mode_ = mode;

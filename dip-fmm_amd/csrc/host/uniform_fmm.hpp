@@ -168,6 +168,13 @@ class UniformFmm {
   // Device-resident operands (K x N x 3 scalars of the plan precision);
   // moments in physical units; ordered after `stream` (hipStream_t or null).
   void evaluate_device(const void* moments, void* fields, int batch, void* stream);
+  // Adjoint (transpose) of the batched field map for reverse-mode
+  // differentiation with respect to the moments: cotangents K x N_t user-order
+  // vectors in, gradients K x N_s x 3 scalars of the plan precision out.  Uses
+  // the identity map of the plan's earlier evaluates (or the fixed one).
+  void evaluate_adjoint_batch_into(std::span<const Vec3> cotangents, int batch, void* gradients);
+  // Device-resident adjoint (K x N_t x 3 in, K x N_s x 3 out).
+  void evaluate_adjoint_device(const void* cotangents, void* gradients, int batch, void* stream);
   // Stage times of the most recent evaluate (waits for a device-resident
   // evaluate to finish).
   EvaluationTimings device_timings();

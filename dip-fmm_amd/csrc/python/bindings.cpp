@@ -248,6 +248,49 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
           py::arg("moments_ptr"), py::arg("fields_ptr"), py::arg("batch"), py::arg("stream") = 0,
           "Device-resident evaluate: pointers to (batch, n, 3) moments and (batch, n_targets, 3) fields of the "
           "plan precision; ordered after `stream`.")
+      .def(
+          "evaluate_adjoint_batch",
+          [](UniformFmm& f, py::object cotangents) -> py::object {
+            const DoubleArray a = py::cast<DoubleArray>(cotangents);
+            const auto info = a.request();
+            if (info.ndim != 3 || info.shape[2] != 3)
+              throw std::invalid_argument("cotangents must have shape (batch, n_targets, 3)");
+            const int K = static_cast<int>(info.shape[0]);
+            const py::ssize_t nt = info.shape[1];
+            std::vector<Vec3> g(static_cast<std::size_t>(K * nt));
+            const double* p = a.data();
+            for (std::size_t i = 0; i < g.size(); ++i) g[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+            const py::ssize_t ns = static_cast<py::ssize_t>(f.tree().sorted_sources().size());
+            if (f.precision() == StaticPrecision::Float32) {
+              py::array_t<float> out({py::ssize_t(K), ns, py::ssize_t(3)});
+              float* o = out.mutable_data();
+              {
+                py::gil_scoped_release release;
+                f.evaluate_adjoint_batch_into(g, K, o);
+              }
+              return out;
+            }
+            py::array_t<double> out({py::ssize_t(K), ns, py::ssize_t(3)});
+            double* o = out.mutable_data();
+            {
+              py::gil_scoped_release release;
+              f.evaluate_adjoint_batch_into(g, K, o);
+            }
+            return out;
+          },
+          py::arg("cotangents"),
+          "Transpose of evaluate_batch: cotangents (batch, n_targets, 3) -> gradients with respect to the "
+          "moments (batch, n_sources, 3).")
+      .def(
+          "evaluate_adjoint_device",
+          [](UniformFmm& f, std::uintptr_t cotangents, std::uintptr_t gradients, int batch, std::uintptr_t stream) {
+            py::gil_scoped_release release;
+            f.evaluate_adjoint_device(reinterpret_cast<const void*>(cotangents), reinterpret_cast<void*>(gradients),
+                                      batch, reinterpret_cast<void*>(stream));
+          },
+          py::arg("cotangents_ptr"), py::arg("gradients_ptr"), py::arg("batch"), py::arg("stream") = 0,
+          "Device-resident adjoint: pointers to (batch, n_targets, 3) cotangents and (batch, n_sources, 3) "
+          "gradients of the plan precision; ordered after `stream`.")
       .def("device_timings", [](UniformFmm& f) {
         EvaluationTimings t;
         {
@@ -263,6 +306,8 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
       .def_property_readonly("coefficient_count", [](const UniformFmm& f) { return f.basis().size(); })
       .def_property_readonly("backend", &UniformFmm::backend)
       .def_property_readonly("precision", &UniformFmm::precision)
+      .def_property_readonly("source_count", [](const UniformFmm& f) { return f.tree().sorted_sources().size(); })
+      .def_property_readonly("target_count", [](const UniformFmm& f) { return f.tree().sorted_targets().size(); })
       .def_property_readonly("coordinate_scale", &UniformFmm::coordinate_scale)
       .def_property_readonly("last_timings", [](const UniformFmm& f) { return timings_dict(f.last_timings()); })
       .def_property_readonly("aggregate_timings", [](const UniformFmm& f) { return timings_dict(f.aggregate_timings()); })

@@ -184,6 +184,26 @@ int hipfmm_plan_create(const hipfmm_plan_desc* desc, hipfmm_plan** plan);
 int hipfmm_plan_evaluate(hipfmm_plan* plan, const void* moments, void* fields, int batch,
                          const int* sorted_self_indices, int memspace, void* stream);
 
+/* Transpose of hipfmm_plan_evaluate: the exact adjoint of the plan's linear
+ * map from moments to fields (same tree, truncation and identity map), for
+ * reverse-mode differentiation with respect to the moments.  No counterpart in
+ * the reference, whose CudaFull path has no adjoint; the transpose it applies
+ * is that of the operators it uses (P2M, M2M, M2L, L2L, L2P, P2P:
+ * src/static_operators.cpp:159-335, :405-470).
+ *   cotangents: batch x target_count x 3 scalars in user target order
+ *   gradients:  batch x source_count x 3 scalars in user source order
+ * With S = diag((-1)^|alpha|), P2M^T = -L2P S, L2L^T = S M2M S and
+ * M2L_o^T = S M2L_-o S (the class matrices are symmetric in alpha, beta and
+ * the level scalings differ by one power of the box width), so the transpose
+ * is the same FMM with the roles of sources and targets exchanged; it runs on
+ * the plan's own buffers.  The identity map is the plan's fixed map (none
+ * before the first evaluate that sets one); a map that pairs two targets with
+ * one source has no transposed form and fails with
+ * HIPFMM_ERROR_INVALID_ARGUMENT.  Memory spaces, scaling and stream ordering
+ * as for hipfmm_plan_evaluate. */
+int hipfmm_plan_evaluate_adjoint(hipfmm_plan* plan, const void* cotangents, void* gradients, int batch,
+                                 int memspace, void* stream);
+
 /* Select the M2L arithmetic of later evaluates (enum hipfmm_m2l_mode). */
 int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode);
 

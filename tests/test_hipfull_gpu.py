@@ -203,6 +203,36 @@ def test_batched_matrix_core_paths_match_fp64_fmm(hip, p, depth, n, K, mode):
         assert e <= 1e-5
 
 
+@pytest.mark.parametrize("p,depth,n,K", [(8, 3, 20000, 1), (8, 4, 30000, 32), (4, 3, 8000, 1)])
+def test_fp16_m2l_is_closer_to_fp64_fmm_than_bf16(hip, p, depth, n, K):
+    # FP16 hi/lo operands with per-degree power-of-two scales on the class
+    # matrices and a per-level power-of-two scale on the multipoles: 22 bits
+    # per operand against bf16's 16, so the far field must come out closer to
+    # the FP64 FMM than the bf16x3 mode's on the same inputs.  The M2L alone
+    # is compared through a far-field-only geometry check: the same points
+    # evaluated with the VALU FP32 M2L (bar 1e-5 for both modes).
+    rng = np.random.default_rng(500 + p + depth + K)
+    x = rng.uniform(0, 1, size=(n, 3))
+    ms = rng.normal(size=(K, n, 3))
+    ids = np.arange(n)
+    announce("fp16 m2l", p, depth, n, K)
+    f = make(hip, x, x, p, depth, 32, ids=ids, mode=hip.HipM2LMode.MFMA_FP16X3)
+    h16 = f.evaluate_batch(ms) if K > 1 else f.evaluate(ms[0])["H"][None]
+    f.set_m2l_mode(hip.HipM2LMode.MFMA_BF16X3)
+    hbf = f.evaluate_batch(ms) if K > 1 else f.evaluate(ms[0])["H"][None]
+    f.set_m2l_mode(hip.HipM2LMode.VALU)
+    h32 = f.evaluate_batch(ms) if K > 1 else f.evaluate(ms[0])["H"][None]
+    ref = O.CpuFmm(x, x, p, depth, 64, fixed_ids=ids)
+    for k in sorted({0, K - 1}):
+        r = ref.evaluate(ms[k])["H"]
+        e16, ebf, e32 = rel_l2(h16[k], r), rel_l2(hbf[k], r), rel_l2(h32[k], r)
+        d16, dbf = rel_l2(h16[k], h32[k]), rel_l2(hbf[k], h32[k])
+        print(f"  state {k}: vs FP64 FMM fp16 {e16:.3e} bf16 {ebf:.3e} valu {e32:.3e}; "
+              f"vs FP32 VALU fp16 {d16:.3e} bf16 {dbf:.3e}", flush=True)
+        assert e16 <= 1e-5
+        assert d16 < dbf
+
+
 def test_batched_near_field_nan_and_self_contract(hip):
     # The matrix-core near field (K = 32) keeps the vector path's contract:
     # the identity pair is skipped and a coincident non-identity pair is NaN.

@@ -219,3 +219,34 @@ def test_m2l_matrices_bit_exact_against_reference():
     for p in (2, 4, 6):
         for off in ((2, 0, 0), (3, -2, 1), (-3, -3, -3), (2, 3, -1)):
             np.testing.assert_array_equal(O.m2l_matrix(p, off), O.m2l_matrix(p, off, kind="ref"))
+
+
+def test_exchanged_roles_fmm_is_the_transpose():
+    # The transposed FMM operator chain (P2M^T = -L2P S, L2L^T = S M2M S,
+    # M2L_o^T = S M2L_-o S with S = diag((-1)^|alpha|)) is the same FMM with
+    # sources and targets exchanged on the same root cube: <y, A x> = <B y, x>
+    # to FP64 rounding, with and without an identity map (inverted for B).
+    rng = np.random.default_rng(3)
+    src, tgt = rng.uniform(0, 1, (3000, 3)), rng.uniform(0.2, 1.2, (2000, 3))
+    f = O.CpuFmm(src, tgt, 5, 3, 64)
+    root = f.root()
+    g = O.CpuFmm(tgt, src, 5, 3, 64, root_centre=root[:3], root_half_width=root[3])
+    x, y = rng.normal(size=(3000, 3)), rng.normal(size=(2000, 3))
+    ax, by = f.evaluate(x)["H"], g.evaluate(y)["H"]
+    assert abs(np.vdot(y, ax) - np.vdot(by, x)) <= 1e-14 * np.linalg.norm(y) * np.linalg.norm(ax)
+    n = 2000
+    pts = rng.uniform(0, 1, (n, 3))
+    perm = rng.permutation(n)
+    f = O.CpuFmm(pts, pts[perm], 4, 2, 64, fixed_ids=perm)
+    root = f.root()
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    g = O.CpuFmm(pts[perm], pts, 4, 2, 64, root_centre=root[:3], root_half_width=root[3], fixed_ids=inv)
+    x, y = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    ax, by = f.evaluate(x)["H"], g.evaluate(y)["H"]
+    assert abs(np.vdot(y, ax) - np.vdot(by, x)) <= 1e-14 * np.linalg.norm(y) * np.linalg.norm(ax)
+    # The identity holds only for the same tree: at another depth B is an
+    # FMM of the transposed problem with its own truncation, off by ~1e-5.
+    g2 = O.CpuFmm(pts[perm], pts, 4, 3, 64, root_centre=root[:3], root_half_width=root[3], fixed_ids=inv)
+    by2 = g2.evaluate(y)["H"]
+    assert abs(np.vdot(y, ax) - np.vdot(by2, x)) > 1e-9 * np.linalg.norm(y) * np.linalg.norm(ax)
