@@ -117,8 +117,10 @@ struct DevicePlan {
   int* self = nullptr;      // nt sorted self indices (-1 none)
   int2* leaf_src = nullptr; // per leaf [begin, end)
   int2* leaf_tgt = nullptr;
-  int2* p2p_tiles = nullptr; // (leaf, first row) of each 128-row near-field item; row = 3*target + component
+  int2* p2p_tiles = nullptr; // (leaf, first row) of each 32-row near-field tile; row = 3*target + component
   int p2p_tile_count = 0;
+  int2* p2p_tiles4 = nullptr; // the same with the potential: row = 4*target + (component or 3 = potential)
+  int p2p_tile4_count = 0;
   // basis tables
   int4* p2m_tab = nullptr;  // per alpha: index(alpha-e_c) or -1 for c=0..2, w = sign (+1/-1)
   // operators
@@ -136,10 +138,13 @@ struct DevicePlan {
   int kcap = 0;
   void* moments_in = nullptr;  // host-path staging: K x ns x 3
   void* fields_out = nullptr;  // host-path staging: K x nt x 3
+  void* potentials_out = nullptr;  // host-path staging: K x nt (with the potential buffers)
   void* sm = nullptr;
   void* M = nullptr;
   void* Lc = nullptr;
   void* Hn = nullptr;
+  void* Pn = nullptr;          // near-field potential Pn[target][K] (allocated once a potential is asked for)
+  double potential_scale = 1.0;  // frame factor of the potential (root width W for FP32 plans)
   // Harmonic-subspace M2L: L = Q sum_o C_o P M with the level scalings folded
   // into P and Q (nc = 0: the M2L runs on the full n coefficients).
   int nc = 0;
@@ -182,9 +187,12 @@ void launch_l2l(const DevicePlan& d, int child_level, int K, hipStream_t s);
 template <class S>
 void launch_m2l_valu(const DevicePlan& d, int K, hipStream_t s);  // all levels 2..L
 template <class S>
-void launch_p2p(const DevicePlan& d, int K, hipStream_t s);
+void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s);
+// Far field from the leaf locals plus the near field, unsorted to user
+// order: fields out[k][target][3] (may be null when potentials are asked
+// for) and potentials pot[k][target] (null: fields only).
 template <class S>
-void launch_l2p_unsort(const DevicePlan& d, S* fields, int K, hipStream_t s);
+void launch_l2p_unsort(const DevicePlan& d, S* fields, S* potentials, int K, hipStream_t s);
 template <class S>
 void launch_zero(S* ptr, size_t count, hipStream_t s);
 

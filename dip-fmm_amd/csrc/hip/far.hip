@@ -330,23 +330,25 @@ __global__ void __launch_bounds__(1024) l2l_kernel(const DevicePlan d, int child
 constexpr int kL2PTargets = 32;  // targets per chunk
 constexpr int kL2PRT = 2;        // targets per thread tile
 
-// L2P layout: tab[n] int4, ex[nm] int4, px[TC][3][p], monoT[nm][TC],
-// Lb[n][KP] (the leaf local).
+// L2P layout: tab[n] int4, ex[NM] int4, px[TC][3][PW], monoT[NM][TC],
+// Lb[n][KP] (the leaf local).  Fields need the NM = nm monomials of degree
+// < p (powers up to p-1); the potential needs all n (powers up to p).
 template <class S>
-MonoLayout l2p_layout(const DevicePlan& d, int KC) {
+MonoLayout l2p_layout(const DevicePlan& d, int KC, bool potential) {
   MonoLayout l{};
   l.NP = round_up(d.n, kRA);
   l.KP = round_up(KC, kRK);
   l.TCP = kL2PTargets;
+  const int NM = potential ? d.n : d.nm, PW = potential ? d.p + 1 : d.p;
   size_t o = 0;
   l.off_tab = o;
   o += align16(size_t(d.n) * sizeof(int4));
   l.off_ex = o;
-  o += align16(size_t(std::max(d.nm, 1)) * sizeof(int4));
+  o += align16(size_t(std::max(NM, 1)) * sizeof(int4));
   l.off_px = o;
-  o += align16(size_t(l.TCP) * 3 * std::max(d.p, 1) * sizeof(S));
+  o += align16(size_t(l.TCP) * 3 * std::max(PW, 1) * sizeof(S));
   l.off_mono = o;
-  o += align16(size_t(std::max(d.nm, 1)) * l.TCP * sizeof(S));
+  o += align16(size_t(std::max(NM, 1)) * l.TCP * sizeof(S));
   l.off_b = o;
   o += align16(size_t(d.n) * l.KP * sizeof(S));
   l.bytes = o;
@@ -354,19 +356,23 @@ MonoLayout l2p_layout(const DevicePlan& d, int KC) {
 }
 
 // One workgroup per target leaf: far field from the leaf local,
-//   H[t][c][k] = -sum_b L[b][k] d_t^(b-e_c)/(b-e_c)!,
-// a thread owning 2 targets x 4 states x 3 components; plus the near field,
-// written to out[k][user target][3].
-template <class S>
-__global__ void __launch_bounds__(1024) l2p_kernel(const DevicePlan d, S* __restrict__ out, int K, int KC,
-                                                   MonoLayout l) {
+//   H[t][c][k] = -sum_b L[b][k] d_t^(b-e_c)/(b-e_c)!
+// (src/static_operators.cpp:300-335), a thread owning 2 targets x 4 states x
+// 3 components; plus the near field, written to out[k][user target][3].
+// With kPot also the potential phi[t][k] = sum_b L[b][k] d_t^b/b! plus the
+// near-field potential, times the frame factor, to pot[k][user target]
+// (out may then be null).
+template <class S, bool kPot>
+__global__ void __launch_bounds__(1024) l2p_kernel(const DevicePlan d, S* __restrict__ out, S* __restrict__ pot, int K,
+                                                   int KC, MonoLayout l) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   int4* tab = reinterpret_cast<int4*>(smem_raw + l.off_tab);
   int4* ex = reinterpret_cast<int4*>(smem_raw + l.off_ex);
   S* px = reinterpret_cast<S*>(smem_raw + l.off_px);
   S* monoT = reinterpret_cast<S*>(smem_raw + l.off_mono);
   S* Lb = reinterpret_cast<S*>(smem_raw + l.off_b);
-  const int n = d.n, nm = d.nm, p = d.p, KP = l.KP;
+  const int n = d.n, KP = l.KP;
+  const int nm = kPot ? d.n : d.nm, p = kPot ? d.p + 1 : d.p;  // monomial count and axis powers held
   constexpr int TC = kL2PTargets;
   const int tid = threadIdx.x, T = blockDim.x;
   const int KT = KP / kRK;
@@ -404,9 +410,10 @@ __global__ void __launch_bounds__(1024) l2p_kernel(const DevicePlan d, S* __rest
     }
     __syncthreads();
     if (tl >= cnt) continue;
-    S acc[3][kL2PRT][kRK];
+    constexpr int C = kPot ? 4 : 3;
+    S acc[C][kL2PRT][kRK];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < C; ++c)
 #pragma unroll
       for (int i = 0; i < kL2PRT; ++i)
 #pragma unroll
@@ -415,9 +422,9 @@ __global__ void __launch_bounds__(1024) l2p_kernel(const DevicePlan d, S* __rest
       const int4 t4 = tab[b];
       S lv[kRK];
       load_vec<S, kRK>(Lb + b * KP + k0, lv);
-      const int idx[3] = {t4.x, t4.y, t4.z};
+      const int idx[4] = {t4.x, t4.y, t4.z, b};
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
+      for (int c = 0; c < C; ++c) {
         if (idx[c] < 0) continue;
         const S* mrow = monoT + idx[c] * TC + tl;
         S mv[kL2PRT];
@@ -439,10 +446,16 @@ __global__ void __launch_bounds__(1024) l2p_kernel(const DevicePlan d, S* __rest
       for (int j = 0; j < kRK; ++j) {
         if (k0 + j >= KCn) break;
         const int k = kc0 + k0 + j;
-        S* o = out + (int64_t(k) * d.nt + user) * 3;
-        o[0] = hn[k] - acc[0][i][j];
-        o[1] = hn[K + k] - acc[1][i][j];
-        o[2] = hn[2 * K + k] - acc[2][i][j];
+        if (!kPot || out != nullptr) {
+          S* o = out + (int64_t(k) * d.nt + user) * 3;
+          o[0] = hn[k] - acc[0][i][j];
+          o[1] = hn[K + k] - acc[1][i][j];
+          o[2] = hn[2 * K + k] - acc[2][i][j];
+        }
+        if constexpr (kPot) {
+          const S pn = static_cast<const S*>(d.Pn)[int64_t(ts) * K + k];
+          pot[int64_t(k) * d.nt + user] = S(d.potential_scale) * (pn + acc[C - 1][i][j]);
+        }
       }
     }
   }
@@ -593,13 +606,19 @@ void launch_l2l(const DevicePlan& d, int child_level, int K, hipStream_t s) {
 }
 
 template <class S>
-void launch_l2p_unsort(const DevicePlan& d, S* fields, int K, hipStream_t s) {
-  if (d.leaves == 0) return;
-  const int kc = state_chunk(K, [&](int c) { return l2p_layout<S>(d, c).bytes; });
-  const MonoLayout l = l2p_layout<S>(d, kc);
+void launch_l2p_unsort(const DevicePlan& d, S* fields, S* potentials, int K, hipStream_t s) {
+  if (d.leaves == 0 || d.nt == 0) return;
+  const bool pot = potentials != nullptr;
+  if (!pot && fields == nullptr) throw invalid_argument("no output requested");
+  if (pot && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
+  const int kc = state_chunk(K, [&](int c) { return l2p_layout<S>(d, c, pot).bytes; });
+  const MonoLayout l = l2p_layout<S>(d, kc, pot);
   const unsigned threads = std::max(tile_threads(l.TCP / kL2PRT, l.KP), unsigned(round_up(3 * l.TCP, 64)));
   const dim3 grid(unsigned(d.leaves), unsigned((K + kc - 1) / kc));
-  l2p_kernel<S><<<grid, threads, l.bytes, s>>>(d, fields, K, kc, l);
+  if (pot)
+    l2p_kernel<S, true><<<grid, threads, l.bytes, s>>>(d, fields, potentials, K, kc, l);
+  else
+    l2p_kernel<S, false><<<grid, threads, l.bytes, s>>>(d, fields, nullptr, K, kc, l);
   HIPFMM_CHECK(hipGetLastError());
 }
 
@@ -650,7 +669,7 @@ void launch_zero(S* ptr, size_t count, hipStream_t s) {
   template void launch_p2m<S>(const DevicePlan&, int, hipStream_t);                \
   template void launch_m2m<S>(const DevicePlan&, int, int, hipStream_t);           \
   template void launch_l2l<S>(const DevicePlan&, int, int, hipStream_t);           \
-  template void launch_l2p_unsort<S>(const DevicePlan&, S*, int, hipStream_t);     \
+  template void launch_l2p_unsort<S>(const DevicePlan&, S*, S*, int, hipStream_t);     \
   template void launch_m2l_compress<S>(const DevicePlan&, int, bool, hipStream_t);  \
   template void launch_m2l_expand<S>(const DevicePlan&, int, hipStream_t);         \
   template void launch_zero<S>(S*, size_t, hipStream_t);

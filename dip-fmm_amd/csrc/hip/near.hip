@@ -57,8 +57,10 @@ __device__ inline double inv_sqrt(double x) { return 1.0 / sqrt(x); }
 
 // Workgroup per target leaf.  Thread = (slot, group); group g handles the k
 // range [kg*KPT, kg*KPT+KPT) and the sources j with j % SP == sp, where
-// KG = number of k groups and SP = 8 / KG source splits.
-template <class S, int KPT>
+// KG = number of k groups and SP = 8 / KG source splits.  With kPot the
+// potential phi += (m.r)|r|^-3/(4 pi) (src/static_operators.cpp:455-463,
+// src/near_field.cpp:30-80) is accumulated beside the field into Pn[t][k].
+template <class S, int KPT, bool kPot>
 __global__ void __launch_bounds__(kThreads) p2p_kernel(const DevicePlan d, int K, int KG, int chunk) {
   extern __shared__ unsigned char smem_raw[];
   auto* spos = reinterpret_cast<PairPos<S>*>(smem_raw);       // chunk
@@ -76,14 +78,17 @@ __global__ void __launch_bounds__(kThreads) p2p_kernel(const DevicePlan d, int K
   const S c4pi = S(0.079577471545947667884441881686257181);  // 1/(4 pi)
   const S* sm = static_cast<const S*>(d.sm);
   S* Hn = static_cast<S*>(d.Hn);
+  constexpr int C = kPot ? 4 : 3;  // accumulated rows per (target, state)
   for (int t0 = trange.x; t0 < trange.y; t0 += kSlots) {
     const int t = t0 + slot;
     const bool active = t < trange.y;
     const PairPos<S> tp = PairPos<S>::load(d.tgt_pos, d.nt, active ? t : trange.x);
     const int self = active ? d.self[t] : -2;
-    S acc[KPT][3];
+    S acc[KPT][C];
 #pragma unroll
-    for (int q = 0; q < KPT; ++q) acc[q][0] = acc[q][1] = acc[q][2] = S(0);
+    for (int q = 0; q < KPT; ++q)
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[q][c] = S(0);
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
@@ -114,6 +119,7 @@ __global__ void __launch_bounds__(kThreads) p2p_kernel(const DevicePlan d, int K
                     acc[q][0] += f * (mu3 * ux - mx);
                     acc[q][1] += f * (mu3 * uy - my);
                     acc[q][2] += f * (mu3 * uz - mz);
+                    if constexpr (kPot) acc[q][C - 1] += f * (rx * mx + ry * my + rz * mz);
                   }
                 }
               }
@@ -124,35 +130,40 @@ __global__ void __launch_bounds__(kThreads) p2p_kernel(const DevicePlan d, int K
     __syncthreads();
     if (SP > 1) {
       for (int q = 0; q < KPT; ++q)
-        for (int c = 0; c < 3; ++c) red[((g * kSlots + slot) * KPT + q) * 3 + c] = acc[q][c];
+        for (int c = 0; c < C; ++c) red[((g * kSlots + slot) * KPT + q) * C + c] = acc[q][c];
       __syncthreads();
       if (sp == 0 && active) {
         for (int s2 = 1; s2 < SP; ++s2) {
           const int g2 = s2 * KG + kg;
           for (int q = 0; q < KPT; ++q)
-            for (int c = 0; c < 3; ++c) acc[q][c] += red[((g2 * kSlots + slot) * KPT + q) * 3 + c];
+            for (int c = 0; c < C; ++c) acc[q][c] += red[((g2 * kSlots + slot) * KPT + q) * C + c];
         }
       }
     }
     if (sp == 0 && active) {
       for (int q = 0; q < KPT; ++q) {
         const int k = kg * KPT + q;
-        if (k < K)
+        if (k < K) {
           for (int c = 0; c < 3; ++c) Hn[(int64_t(t) * 3 + c) * K + k] = acc[q][c];
+          if constexpr (kPot) static_cast<S*>(d.Pn)[int64_t(t) * K + k] = acc[q][C - 1];
+        }
       }
     }
   }
 }
 
 template <class S, int KPT>
-void launch_p2p_t(const DevicePlan& d, int K, hipStream_t s) {
+void launch_p2p_t(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   int KG = (K + KPT - 1) / KPT;
   int kg = 1;
   while (kg < KG) kg *= 2;  // 1, 2, 4 or 8 k groups
   const int chunk = sizeof(S) == 4 ? 64 : 32;
   const size_t stage = size_t(chunk) * sizeof(PairPos<S>) + size_t(chunk) * 3 * K * sizeof(S);
-  const size_t reduce = size_t(8) * kSlots * KPT * 3 * sizeof(S) + size_t(chunk) * sizeof(PairPos<S>);
-  p2p_kernel<S, KPT><<<unsigned(d.leaves), kThreads, std::max(stage, reduce), s>>>(d, K, kg, chunk);
+  const size_t reduce = size_t(8) * kSlots * KPT * 4 * sizeof(S) + size_t(chunk) * sizeof(PairPos<S>);
+  if (potential)
+    p2p_kernel<S, KPT, true><<<unsigned(d.leaves), kThreads, std::max(stage, reduce), s>>>(d, K, kg, chunk);
+  else
+    p2p_kernel<S, KPT, false><<<unsigned(d.leaves), kThreads, std::max(stage, reduce), s>>>(d, K, kg, chunk);
   HIPFMM_CHECK(hipGetLastError());
 }
 
@@ -210,7 +221,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // states the near field of a leaf is one GEMM:
 //   H[(t,c)][k] = sum_{(s,c')} D_{cc'}(t,s) m[s][c'][k],
 // rows (target, component) packed 3 per target (row = 3*t_local + c), depth
-// (source, component), columns k.  One wave owns a 32-row tile: each lane
+// (source, component), columns k.  With the potential (kRows = 4) a fourth
+// row per target carries phi = sum_{c'} f r_{c'} m[s][c'][k].  One wave owns
+// a 32-row tile: each lane
 // forms the pair tensor entries D_{c,0..2}(t,s) of its row's target and one
 // of the two sources of a step (A operand, lane i: row i%32, depth i/32) and
 // loads m[s][c'][k] (B operand, lane i: column i%32, depth i/32); the
@@ -218,19 +231,22 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // NaN as in the vector path.  Waves are independent: no LDS, no barriers.
 constexpr int kP2PWaves = 4;
 
-__global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePlan d, int K) {
+template <int kRows>
+__global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePlan d, const int2* __restrict__ tiles,
+                                                                   int tile_count, int K) {
   const int wave = int(threadIdx.x) >> 6, lane = threadIdx.x & 63;
   const int item = int(blockIdx.x) * kP2PWaves + wave;
-  if (item >= d.p2p_tile_count) return;
-  const int2 tile = d.p2p_tiles[item];
+  if (item >= tile_count) return;
+  const int2 tile = tiles[item];
   const int leaf = tile.x;
   const int2 trange = d.leaf_tgt[leaf];
-  const int rows = 3 * (trange.y - trange.x);
+  const int rows = kRows * (trange.y - trange.x);
   const int col = lane & 31, g = lane >> 5;
   const int row = tile.y + col;  // A-operand row of this lane
   const bool row_ok = row < rows;
-  const int t = trange.x + (row_ok ? row / 3 : 0);
-  const int c = row_ok ? row % 3 : 0;
+  const int t = trange.x + (row_ok ? row / kRows : 0);
+  const int c = row_ok ? row % kRows : 0;
+  const bool pot = kRows == 4 && c == 3;  // potential row
   const PairPos<float> tp = PairPos<float>::load(d.tgt_pos, d.nt, t);
   const int self = row_ok ? d.self[t] : -2;
   const int k = int(blockIdx.y) * 32 + col;  // B / C column of this lane
@@ -270,9 +286,17 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePl
           const float uc = c == 0 ? ux : (c == 1 ? uy : uz);
           const float f3uc = 3.0f * f * uc;
           const bool use = row_ok & src_ok & (s != self);
-          const float a0 = use ? f3uc * ux - (c == 0 ? f : 0.0f) : 0.0f;
-          const float a1 = use ? f3uc * uy - (c == 1 ? f : 0.0f) : 0.0f;
-          const float a2 = use ? f3uc * uz - (c == 2 ? f : 0.0f) : 0.0f;
+          float a0 = f3uc * ux - (c == 0 ? f : 0.0f);
+          float a1 = f3uc * uy - (c == 1 ? f : 0.0f);
+          float a2 = f3uc * uz - (c == 2 ? f : 0.0f);
+          if constexpr (kRows == 4) {
+            a0 = pot ? f * rx : a0;
+            a1 = pot ? f * ry : a1;
+            a2 = pot ? f * rz : a2;
+          }
+          a0 = use ? a0 : 0.0f;
+          a1 = use ? a1 : 0.0f;
+          a2 = use ? a2 : 0.0f;
           const float* ms = sm + int64_t(sl) * 3 * K + k;
           const float b0 = src_ok ? ms[0] : 0.0f;
           const float b1 = src_ok ? ms[K] : 0.0f;
@@ -290,31 +314,46 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePl
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int orow = tile.y + 8 * (r >> 2) + 4 * g + (r & 3);
-    if (orow < rows) Hn[(int64_t(trange.x) * 3 + orow) * K + k] = acc[r];
+    if (orow >= rows) continue;
+    if constexpr (kRows == 3) {
+      Hn[(int64_t(trange.x) * 3 + orow) * K + k] = acc[r];
+    } else {
+      const int64_t ot = trange.x + orow / 4;
+      const int oc = orow % 4;
+      if (oc < 3)
+        Hn[(ot * 3 + oc) * K + k] = acc[r];
+      else
+        static_cast<float*>(d.Pn)[ot * K + k] = acc[r];
+    }
   }
 }
 
 template <class S>
-void launch_p2p(const DevicePlan& d, int K, hipStream_t s) {
+void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
+  if (potential && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
   if constexpr (sizeof(S) == 4) {
     if (K % 32 == 0) {
-      if (d.p2p_tile_count == 0) return;
-      const dim3 grid(unsigned((d.p2p_tile_count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
-      p2p_mfma_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, K);
+      const int count = potential ? d.p2p_tile4_count : d.p2p_tile_count;
+      if (count == 0) return;
+      const dim3 grid(unsigned((count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
+      if (potential)
+        p2p_mfma_kernel<4><<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles4, count, K);
+      else
+        p2p_mfma_kernel<3><<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles, count, K);
       HIPFMM_CHECK(hipGetLastError());
       return;
     }
   }
   if (K <= 8)
-    launch_p2p_t<S, 1>(d, K, s);
+    launch_p2p_t<S, 1>(d, K, potential, s);
   else if (K <= 32)
-    launch_p2p_t<S, 4>(d, K, s);
+    launch_p2p_t<S, 4>(d, K, potential, s);
   else
-    launch_p2p_t<S, 8>(d, K, s);
+    launch_p2p_t<S, 8>(d, K, potential, s);
 }
-template void launch_p2p<float>(const DevicePlan&, int, hipStream_t);
-template void launch_p2p<double>(const DevicePlan&, int, hipStream_t);
+template void launch_p2p<float>(const DevicePlan&, int, bool, hipStream_t);
+template void launch_p2p<double>(const DevicePlan&, int, bool, hipStream_t);
 
 void direct_field_fp64(const double* targets, int nt, const double* sources, int ns, const double* moments,
                        const int* self, double* fields) {

@@ -66,6 +66,9 @@ struct hipfmm_plan {
   int adj_tile_count = 0;
   int* adj_self = nullptr;
   bool adj_self_valid = true;  // false when two targets share one source
+  bool potential_used = false;  // keep the near-field potential buffers in the state
+  void* pinned_pot = nullptr;
+  size_t pinned_pot_bytes = 0;
   double device_moment_scale = 1.0;
   int m2l_mode = HIPFMM_M2L_VALU;
   bool mfma_ready = false;
@@ -310,6 +313,12 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
     for (int r0 = 0; r0 < 3 * (lt[leaf].y - lt[leaf].x); r0 += 32) tiles.push_back(int2{int(leaf), r0});
   d.p2p_tile_count = int(tiles.size());
   d.p2p_tiles = upload(P, tiles.data(), std::max<size_t>(tiles.size(), 1));
+  std::vector<int2> tiles4;
+  for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
+    for (int r0 = 0; r0 < 4 * (lt[leaf].y - lt[leaf].x); r0 += 32) tiles4.push_back(int2{int(leaf), r0});
+  d.p2p_tile4_count = int(tiles4.size());
+  d.p2p_tiles4 = upload(P, tiles4.data(), std::max<size_t>(tiles4.size(), 1));
+  d.potential_scale = desc.potential_scale > 0.0 ? desc.potential_scale : 1.0;
   std::vector<int2> adj_tiles;
   for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
     for (int r0 = 0; r0 < 3 * (ls[leaf].y - ls[leaf].x); r0 += 32) adj_tiles.push_back(int2{int(leaf), r0});
@@ -492,8 +501,12 @@ void free_state(hipfmm_plan* P) {
 }
 
 template <class S>
-void ensure_capacity(hipfmm_plan* P, int K) {
+void ensure_capacity(hipfmm_plan* P, int K, bool potential) {
   DevicePlan& d = P->d;
+  if (potential && !P->potential_used) {
+    P->potential_used = true;
+    free_state(P);  // reallocated below with the potential buffers
+  }
   if (K <= d.kcap) return;
   free_state(P);
   const int cap = std::min(hipfmm::kMaxBatch, std::max(K, 1));
@@ -508,6 +521,12 @@ void ensure_capacity(hipfmm_plan* P, int K) {
   d.fields_out = alloc(points * 3 * cap, sizeof(S));
   d.sm = alloc(points * 3 * cap, sizeof(S));
   d.Hn = alloc(points * 3 * cap, sizeof(S));
+  d.Pn = nullptr;
+  d.potentials_out = nullptr;
+  if (P->potential_used) {
+    d.Pn = alloc(points * cap, sizeof(S));
+    d.potentials_out = alloc(points * cap, sizeof(S));
+  }
   d.M = alloc(cs, sizeof(S));
   d.Lc = alloc(cs, sizeof(S));
   HIPFMM_CHECK(hipMemset(d.M, 0, cs * sizeof(S)));
@@ -578,14 +597,17 @@ DevicePlan role_exchanged(const hipfmm_plan* P) {
 }
 
 template <class S>
-void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, const int* self, int memspace,
-                   hipStream_t user, bool adjoint) {
+void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* potentials, int K, const int* self,
+                   int memspace, hipStream_t user, bool adjoint) {
+  const bool want_pot = potentials != nullptr;
   {
     const DevicePlan& f = P->d;
     const int n_in = adjoint ? f.nt : f.ns, n_out = adjoint ? f.ns : f.nt;
     require(K >= 1 && K <= hipfmm::kMaxBatch, "batch must be between 1 and 64 moment states");
     require(moments != nullptr || n_in == 0, adjoint ? "cotangents pointer is null" : "moments pointer is null");
-    require(fields != nullptr || n_out == 0, adjoint ? "gradients pointer is null" : "fields pointer is null");
+    require(!adjoint || potentials == nullptr, "the adjoint is of the field map only");
+    require(want_pot || fields != nullptr || n_out == 0,
+            adjoint ? "gradients pointer is null" : "fields pointer is null");
     require(memspace == HIPFMM_HOST || memspace == HIPFMM_DEVICE, "unknown memory space");
     // Identity map: fixed with the plan on first use (src/cuda_fmm.cu:3470-3486).
     if (self != nullptr) {
@@ -606,17 +628,25 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
   }
   if (P->timings_pending) finalize_timings(P);
   (void)hipGetLastError();
-  ensure_capacity<S>(P, K);
+  ensure_capacity<S>(P, K, want_pot);
   const DevicePlan d = adjoint ? role_exchanged(P) : P->d;
   hipStream_t s = P->s_far;
   hipEvent_t* e = P->ev;
   const S* mu = nullptr;
   S* out = nullptr;
+  S* pot = nullptr;
   S scale = S(1);
   const size_t in_bytes = size_t(K) * d.ns * 3 * sizeof(S);
-  const size_t out_bytes = size_t(K) * d.nt * 3 * sizeof(S);
+  const size_t out_bytes = fields != nullptr ? size_t(K) * d.nt * 3 * sizeof(S) : 0;
+  const size_t pot_bytes = want_pot ? size_t(K) * d.nt * sizeof(S) : 0;
   const bool host = memspace == HIPFMM_HOST;
   if (host) {
+    if (P->pinned_pot_bytes < pot_bytes) {
+      if (P->pinned_pot) (void)hipHostFree(P->pinned_pot);
+      P->pinned_pot = nullptr;
+      HIPFMM_CHECK(hipHostMalloc(&P->pinned_pot, std::max<size_t>(pot_bytes, 16)));
+      P->pinned_pot_bytes = pot_bytes;
+    }
     if (P->pinned_in_bytes < in_bytes) {
       if (P->pinned_in) (void)hipHostFree(P->pinned_in);
       P->pinned_in = nullptr;
@@ -634,9 +664,10 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
     if (in_bytes) HIPFMM_CHECK(hipMemcpyAsync(d.moments_in, P->pinned_in, in_bytes, hipMemcpyHostToDevice, s));
     HIPFMM_CHECK(hipEventRecord(e[kEvH2D1], s));
     mu = static_cast<const S*>(d.moments_in);
-    out = static_cast<S*>(d.fields_out);
+    out = fields != nullptr ? static_cast<S*>(d.fields_out) : nullptr;
+    pot = want_pot ? static_cast<S*>(d.potentials_out) : nullptr;
     P->stats.evaluation_h2d_bytes = in_bytes;
-    P->stats.evaluation_d2h_bytes = out_bytes;
+    P->stats.evaluation_d2h_bytes = out_bytes + pot_bytes;
     P->stats.evaluation_h2d_calls += 1;
     P->stats.evaluation_d2h_calls += 1;
   } else {
@@ -646,6 +677,7 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
     HIPFMM_CHECK(hipEventRecord(e[kEvH2D1], s));
     mu = static_cast<const S*>(moments);
     out = static_cast<S*>(fields);
+    pot = static_cast<S*>(potentials);
     scale = S(P->device_moment_scale);
     P->stats.evaluation_h2d_bytes = 0;
     P->stats.evaluation_d2h_bytes = 0;
@@ -654,7 +686,7 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
   HIPFMM_CHECK(hipEventRecord(e[kEvGather], s));
   HIPFMM_CHECK(hipStreamWaitEvent(P->s_near, e[kEvGather], 0));
   HIPFMM_CHECK(hipEventRecord(e[kEvNear0], P->s_near));
-  hipfmm::launch_p2p<S>(d, K, P->s_near);
+  hipfmm::launch_p2p<S>(d, K, want_pot, P->s_near);
   HIPFMM_CHECK(hipEventRecord(e[kEvNear1], P->s_near));
   hipfmm::launch_p2m<S>(d, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvP2M], s));
@@ -693,7 +725,7 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
   for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(d, level, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvL2L], s));
   HIPFMM_CHECK(hipStreamWaitEvent(s, e[kEvNear1], 0));
-  hipfmm::launch_l2p_unsort<S>(d, out, K, s);
+  hipfmm::launch_l2p_unsort<S>(d, out, pot, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvL2P], s));
   P->last_K = K;
   P->last_host = host;
@@ -701,9 +733,11 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, int K, con
   if (host) {
     HIPFMM_CHECK(hipEventRecord(e[kEvD2H0], s));
     if (out_bytes) HIPFMM_CHECK(hipMemcpyAsync(P->pinned_out, d.fields_out, out_bytes, hipMemcpyDeviceToHost, s));
+    if (pot_bytes) HIPFMM_CHECK(hipMemcpyAsync(P->pinned_pot, d.potentials_out, pot_bytes, hipMemcpyDeviceToHost, s));
     HIPFMM_CHECK(hipEventRecord(e[kEvD2H1], s));
     HIPFMM_CHECK(hipStreamSynchronize(s));
     if (out_bytes) std::memcpy(fields, P->pinned_out, out_bytes);
+    if (pot_bytes) std::memcpy(potentials, P->pinned_pot, pot_bytes);
     finalize_timings(P);
   } else {
     HIPFMM_CHECK(hipStreamWaitEvent(user, e[kEvL2P], 0));
@@ -801,9 +835,24 @@ int hipfmm_plan_evaluate(hipfmm_plan* plan, const void* moments, void* fields, i
     DeviceGuard guard(plan->device);
     const hipStream_t user = static_cast<hipStream_t>(stream);
     if (plan->d.precision == 32)
-      evaluate_impl<float>(plan, moments, fields, batch, sorted_self_indices, memspace, user, false);
+      evaluate_impl<float>(plan, moments, fields, nullptr, batch, sorted_self_indices, memspace, user, false);
     else
-      evaluate_impl<double>(plan, moments, fields, batch, sorted_self_indices, memspace, user, false);
+      evaluate_impl<double>(plan, moments, fields, nullptr, batch, sorted_self_indices, memspace, user, false);
+  });
+}
+
+int hipfmm_plan_evaluate_potential(hipfmm_plan* plan, const void* moments, void* fields, void* potentials, int batch,
+                                   const int* sorted_self_indices, int memspace, void* stream) {
+  return guarded([&] {
+    require(plan != nullptr, "plan is null");
+    require(potentials != nullptr || plan->d.nt == 0, "potentials pointer is null");
+    if (potentials == nullptr && fields == nullptr) return;  // no targets, nothing asked for
+    DeviceGuard guard(plan->device);
+    const hipStream_t user = static_cast<hipStream_t>(stream);
+    if (plan->d.precision == 32)
+      evaluate_impl<float>(plan, moments, fields, potentials, batch, sorted_self_indices, memspace, user, false);
+    else
+      evaluate_impl<double>(plan, moments, fields, potentials, batch, sorted_self_indices, memspace, user, false);
   });
 }
 
@@ -814,9 +863,9 @@ int hipfmm_plan_evaluate_adjoint(hipfmm_plan* plan, const void* cotangents, void
     DeviceGuard guard(plan->device);
     const hipStream_t user = static_cast<hipStream_t>(stream);
     if (plan->d.precision == 32)
-      evaluate_impl<float>(plan, cotangents, gradients, batch, nullptr, memspace, user, true);
+      evaluate_impl<float>(plan, cotangents, gradients, nullptr, batch, nullptr, memspace, user, true);
     else
-      evaluate_impl<double>(plan, cotangents, gradients, batch, nullptr, memspace, user, true);
+      evaluate_impl<double>(plan, cotangents, gradients, nullptr, batch, nullptr, memspace, user, true);
   });
 }
 
@@ -883,6 +932,7 @@ void hipfmm_plan_destroy(hipfmm_plan* plan) {
   plan->persistent.clear();
   if (plan->pinned_in) (void)hipHostFree(plan->pinned_in);
   if (plan->pinned_out) (void)hipHostFree(plan->pinned_out);
+  if (plan->pinned_pot) (void)hipHostFree(plan->pinned_pot);
   for (auto& ev : plan->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (plan->s_far) (void)hipStreamDestroy(plan->s_far);

@@ -379,6 +379,7 @@ void UniformFmm::build_plan(const UniformFmmOptions&) {
   desc.l2l_operators = l2l.data();
   desc.fixed_self_indices = fixed_self.empty() ? nullptr : fixed_self.data();
   desc.device_moment_scale = fp32 ? 1.0 / (scale_ * scale_ * scale_) : 1.0;
+  desc.potential_scale = fp32 ? scale_ : 1.0;
   desc.m2l_mode = static_cast<int>(m2l_mode_);
   check(hipfmm_plan_create(&desc, &plan_));
   setup_seconds_ = std::chrono::duration<double>(Clock::now() - start).count();
@@ -460,7 +461,6 @@ void UniformFmm::evaluate_into_float32(std::span<const Vec3> moments, std::span<
   if (results.size() != nt)
     throw std::invalid_argument("UniformFmm::evaluate_into_float32 requires one result per target");
   validate_identities(ids, moments.size());
-  if (output != OutputFlags::Field) throw std::invalid_argument("HipFull currently supports field-only evaluation");
   if (moments.size() != tree_.sorted_sources().size())
     throw std::invalid_argument("full FP32 HipFull FMM dimensions are inconsistent");
   const auto self = sorted_self_indices(ids);
@@ -471,10 +471,31 @@ void UniformFmm::evaluate_into_float32(std::span<const Vec3> moments, std::span<
     m[3 * i + 1] = static_cast<float>(v.y / scale_ / scale_ / scale_);
     m[3 * i + 2] = static_cast<float>(v.z / scale_ / scale_ / scale_);
   }
-  std::vector<float> f(3 * nt);
-  check(hipfmm_plan_evaluate(plan_, m.data(), f.data(), 1, self.data(), HIPFMM_HOST, nullptr));
-  for (std::size_t t = 0; t < nt; ++t) results[t] = {0.0F, {f[3 * t], f[3 * t + 1], f[3 * t + 2]}};
-  record_timings();
+  std::vector<float> f, phi;
+  run_outputs(m.data(), f, phi, output, self);
+  for (std::size_t t = 0; t < nt; ++t)
+    results[t] = {phi.empty() ? 0.0F : phi[t],
+                  f.empty() ? FloatVec3{0.0F, 0.0F, 0.0F} : FloatVec3{f[3 * t], f[3 * t + 1], f[3 * t + 2]}};
+}
+
+// One host evaluate with the outputs `output` asks for: the field entry
+// point, or the potential entry point (fields optional), or nothing
+// (OutputFlags::None: all-zero results, src/uniform_fmm.cpp:1389-1437 write
+// neither output).
+template <class S>
+void UniformFmm::run_outputs(const S* moments, std::vector<S>& fields, std::vector<S>& potentials,
+                             OutputFlags output, const std::vector<int>& self) {
+  const std::size_t nt = tree_.sorted_targets().size();
+  const bool want_field = has_flag(output, OutputFlags::Field);
+  const bool want_pot = has_flag(output, OutputFlags::Potential);
+  if (want_field) fields.assign(3 * nt, S(0));
+  if (want_pot) potentials.assign(nt, S(0));
+  if (want_pot)
+    check(hipfmm_plan_evaluate_potential(plan_, moments, want_field ? fields.data() : nullptr, potentials.data(), 1,
+                                         self.data(), HIPFMM_HOST, nullptr));
+  else if (want_field)
+    check(hipfmm_plan_evaluate(plan_, moments, fields.data(), 1, self.data(), HIPFMM_HOST, nullptr));
+  if (want_field || want_pot) record_timings();
 }
 
 std::vector<FloatPotentialField> UniformFmm::evaluate_float32(std::span<const Vec3> moments, OutputFlags output,
@@ -499,7 +520,6 @@ void UniformFmm::evaluate_into(std::span<const Vec3> moments, std::span<Potentia
   ids = resolve_self_indices(ids);
   if (results.size() != nt) throw std::invalid_argument("UniformFmm::evaluate_into requires one result per target");
   validate_identities(ids, moments.size());
-  if (output != OutputFlags::Field) throw std::invalid_argument("HipFull currently supports field-only evaluation");
   if (moments.size() != tree_.sorted_sources().size())
     throw std::invalid_argument("full FP64 HipFull FMM dimensions are inconsistent");
   const auto self = sorted_self_indices(ids);
@@ -509,10 +529,10 @@ void UniformFmm::evaluate_into(std::span<const Vec3> moments, std::span<Potentia
     m[3 * i + 1] = moments[i].y;
     m[3 * i + 2] = moments[i].z;
   }
-  std::vector<double> f(3 * nt);
-  check(hipfmm_plan_evaluate(plan_, m.data(), f.data(), 1, self.data(), HIPFMM_HOST, nullptr));
-  for (std::size_t t = 0; t < nt; ++t) results[t] = {0.0, {f[3 * t], f[3 * t + 1], f[3 * t + 2]}};
-  record_timings();
+  std::vector<double> f, phi;
+  run_outputs(m.data(), f, phi, output, self);
+  for (std::size_t t = 0; t < nt; ++t)
+    results[t] = {phi.empty() ? 0.0 : phi[t], f.empty() ? Vec3{0.0, 0.0, 0.0} : Vec3{f[3 * t], f[3 * t + 1], f[3 * t + 2]}};
 }
 
 std::vector<PotentialField> UniformFmm::evaluate(std::span<const Vec3> moments, OutputFlags output,
@@ -531,6 +551,11 @@ std::vector<PotentialField> UniformFmm::evaluate_float64(std::span<const Vec3> m
 
 void UniformFmm::evaluate_batch_into(std::span<const Vec3> moments, int batch, void* fields,
                                      std::span<const int> ids) {
+  evaluate_batch_potential_into(moments, batch, fields, nullptr, ids);
+}
+
+void UniformFmm::evaluate_batch_potential_into(std::span<const Vec3> moments, int batch, void* fields,
+                                               void* potentials, std::span<const int> ids) {
   const std::size_t ns = tree_.sorted_sources().size();
   if (batch < 1 || moments.size() != ns * static_cast<std::size_t>(batch))
     throw std::invalid_argument("batched moments must hold batch x source_count vectors");
@@ -544,7 +569,9 @@ void UniformFmm::evaluate_batch_into(std::span<const Vec3> moments, int batch, v
       m[3 * i + 1] = static_cast<float>(moments[i].y / scale_ / scale_ / scale_);
       m[3 * i + 2] = static_cast<float>(moments[i].z / scale_ / scale_ / scale_);
     }
-    check(hipfmm_plan_evaluate(plan_, m.data(), fields, batch, self.data(), HIPFMM_HOST, nullptr));
+    check(potentials ? hipfmm_plan_evaluate_potential(plan_, m.data(), fields, potentials, batch, self.data(),
+                                                      HIPFMM_HOST, nullptr)
+                     : hipfmm_plan_evaluate(plan_, m.data(), fields, batch, self.data(), HIPFMM_HOST, nullptr));
   } else {
     std::vector<double> m(3 * moments.size());
     for (std::size_t i = 0; i < moments.size(); ++i) {
@@ -552,7 +579,9 @@ void UniformFmm::evaluate_batch_into(std::span<const Vec3> moments, int batch, v
       m[3 * i + 1] = moments[i].y;
       m[3 * i + 2] = moments[i].z;
     }
-    check(hipfmm_plan_evaluate(plan_, m.data(), fields, batch, self.data(), HIPFMM_HOST, nullptr));
+    check(potentials ? hipfmm_plan_evaluate_potential(plan_, m.data(), fields, potentials, batch, self.data(),
+                                                      HIPFMM_HOST, nullptr)
+                     : hipfmm_plan_evaluate(plan_, m.data(), fields, batch, self.data(), HIPFMM_HOST, nullptr));
   }
   record_timings();
 }
@@ -565,6 +594,17 @@ void UniformFmm::evaluate_device(const void* moments, void* fields, int batch, v
     self = s.data();
   }
   check(hipfmm_plan_evaluate(plan_, moments, fields, batch, self, HIPFMM_DEVICE, stream));
+}
+
+void UniformFmm::evaluate_potential_device(const void* moments, void* fields, void* potentials, int batch,
+                                           void* stream) {
+  const int* self = nullptr;
+  std::vector<int> s;
+  if (fixed_ids_.has_value()) {
+    s = sorted_self_indices(fixed_ids_.value());
+    self = s.data();
+  }
+  check(hipfmm_plan_evaluate_potential(plan_, moments, fields, potentials, batch, self, HIPFMM_DEVICE, stream));
 }
 
 void UniformFmm::evaluate_adjoint_batch_into(std::span<const Vec3> cotangents, int batch, void* gradients) {

@@ -40,6 +40,9 @@ struct FloatPotentialField {
 
 // include/cdfmm/output_flags.hpp:12-17
 enum class OutputFlags : unsigned { None = 0u, Potential = 1u, Field = 2u, Both = 3u };
+inline bool has_flag(OutputFlags value, OutputFlags flag) {
+  return (static_cast<unsigned>(value) & static_cast<unsigned>(flag)) != 0u;
+}
 // include/cdfmm/precision.hpp:11-14
 enum class StaticPrecision { Float32, Float64 };
 // include/cdfmm/uniform_fmm.hpp:22-31, plus HipFull.
@@ -165,6 +168,12 @@ class UniformFmm {
   // fields: K x N_t x 3 scalars of the plan precision (float or double).
   void evaluate_batch_into(std::span<const Vec3> moments, int batch, void* fields,
                            std::span<const int> target_source_indices = {});
+  // The same with potentials: K x N_t scalars of the plan precision
+  // (required); fields may be null for potentials only.
+  void evaluate_batch_potential_into(std::span<const Vec3> moments, int batch, void* fields, void* potentials,
+                                     std::span<const int> target_source_indices = {});
+  // Device-resident fields (or null) and potentials.
+  void evaluate_potential_device(const void* moments, void* fields, void* potentials, int batch, void* stream);
   // Device-resident operands (K x N x 3 scalars of the plan precision);
   // moments in physical units; ordered after `stream` (hipStream_t or null).
   void evaluate_device(const void* moments, void* fields, int batch, void* stream);
@@ -202,6 +211,9 @@ class UniformFmm {
   void validate_identities(std::span<const int> ids, std::size_t moment_count) const;
   std::vector<int> sorted_self_indices(std::span<const int> ids) const;
   void record_timings();
+  template <class S>
+  void run_outputs(const S* moments, std::vector<S>& fields, std::vector<S>& potentials, OutputFlags output,
+                   const std::vector<int>& self);
 
   UniformTree tree_;
   Basis basis_;

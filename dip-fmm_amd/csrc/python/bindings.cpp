@@ -249,6 +249,59 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
           "Device-resident evaluate: pointers to (batch, n, 3) moments and (batch, n_targets, 3) fields of the "
           "plan precision; ordered after `stream`.")
       .def(
+          "evaluate_batch_outputs",
+          [](UniformFmm& f, py::object moments, const std::string& output, py::object ids) -> py::dict {
+            const DoubleArray a = py::cast<DoubleArray>(moments);
+            const auto info = a.request();
+            if (info.ndim != 3 || info.shape[2] != 3)
+              throw std::invalid_argument("dipole_moments must have shape (batch, n, 3)");
+            const OutputFlags flags = parse_output(output);
+            const bool want_field = has_flag(flags, OutputFlags::Field);
+            if (!has_flag(flags, OutputFlags::Potential))
+              throw std::invalid_argument("evaluate_batch_outputs is for 'potential' or 'both'; use evaluate_batch");
+            const int K = static_cast<int>(info.shape[0]);
+            std::vector<Vec3> m(static_cast<std::size_t>(K * info.shape[1]));
+            const double* p = a.data();
+            for (std::size_t i = 0; i < m.size(); ++i) m[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+            std::vector<int> id;
+            if (!ids.is_none()) id = py::cast<std::vector<int>>(ids);
+            const py::ssize_t nt = static_cast<py::ssize_t>(f.tree().sorted_targets().size());
+            py::dict out;
+            auto run = [&](auto tag) {
+              using S = decltype(tag);
+              py::array_t<S> phi({py::ssize_t(K), nt});
+              py::array_t<S> H(want_field ? std::vector<py::ssize_t>{py::ssize_t(K), nt, py::ssize_t(3)}
+                                          : std::vector<py::ssize_t>{py::ssize_t(0)});
+              S* hp = want_field ? H.mutable_data() : nullptr;
+              S* pp = phi.mutable_data();
+              {
+                py::gil_scoped_release release;
+                f.evaluate_batch_potential_into(m, K, hp, pp, id);
+              }
+              out["phi"] = phi;
+              out["H"] = want_field ? py::object(H) : py::none();
+            };
+            if (f.precision() == StaticPrecision::Float32)
+              run(float{});
+            else
+              run(double{});
+            return out;
+          },
+          py::arg("dipole_moments"), py::arg("output") = "both", py::arg("target_source_indices") = py::none(),
+          "Potentials (and fields with output='both') of K moment states: {'phi': (batch, n_targets), "
+          "'H': (batch, n_targets, 3) or None}.")
+      .def(
+          "evaluate_potential_device",
+          [](UniformFmm& f, std::uintptr_t moments, std::uintptr_t fields, std::uintptr_t potentials, int batch,
+             std::uintptr_t stream) {
+            py::gil_scoped_release release;
+            f.evaluate_potential_device(reinterpret_cast<const void*>(moments), reinterpret_cast<void*>(fields),
+                                        reinterpret_cast<void*>(potentials), batch, reinterpret_cast<void*>(stream));
+          },
+          py::arg("moments_ptr"), py::arg("fields_ptr"), py::arg("potentials_ptr"), py::arg("batch"),
+          py::arg("stream") = 0,
+          "Device-resident potentials (batch, n_targets) and, unless fields_ptr is 0, fields (batch, n_targets, 3).")
+      .def(
           "evaluate_adjoint_batch",
           [](UniformFmm& f, py::object cotangents) -> py::object {
             const DoubleArray a = py::cast<DoubleArray>(cotangents);

@@ -102,6 +102,10 @@ typedef struct hipfmm_plan_desc {
    * scaled already and use factor 1. */
   double device_moment_scale;
   int m2l_mode; /* enum hipfmm_m2l_mode */
+  /* Factor applied to every potential before it is returned: the root width
+   * W for FP32 plans, whose potential is computed in the normalised frame
+   * (src/uniform_fmm.cpp:1448-1451); 1 for FP64 plans.  0 reads as 1. */
+  double potential_scale;
 } hipfmm_plan_desc;
 
 /* Host/device traffic and storage; mirrors CudaPlanStatistics
@@ -183,6 +187,19 @@ int hipfmm_plan_create(const hipfmm_plan_desc* desc, hipfmm_plan** plan);
  * before any later work on `stream`. */
 int hipfmm_plan_evaluate(hipfmm_plan* plan, const void* moments, void* fields, int batch,
                          const int* sorted_self_indices, int memspace, void* stream);
+
+/* Fields and potentials of `batch` moment states: CpuStatic's Potential /
+ * Both outputs (src/uniform_fmm.cpp:1414-1437 potential rows, near field
+ * src/near_field.cpp:30-80) on the device; the reference's CudaFull is
+ * field-only (src/uniform_fmm.cpp:1103-1106).
+ *   potentials: batch x target_count scalars in user target order (required)
+ *   fields:     batch x target_count x 3 scalars, or NULL for potentials only
+ * Everything else as for hipfmm_plan_evaluate.  The potential is the far
+ * field sum_b L_b d^b/b! of the leaf local plus the near-field
+ * (m.r)/(4 pi |r|^3), the identity pair skipped and coincident non-identity
+ * pairs NaN, times desc.potential_scale. */
+int hipfmm_plan_evaluate_potential(hipfmm_plan* plan, const void* moments, void* fields, void* potentials, int batch,
+                                   const int* sorted_self_indices, int memspace, void* stream);
 
 /* Transpose of hipfmm_plan_evaluate: the exact adjoint of the plan's linear
  * map from moments to fields (same tree, truncation and identity map), for

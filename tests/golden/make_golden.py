@@ -16,6 +16,10 @@ Every fixture holds inputs and the compiled reference's outputs:
   empty leaves, separate targets, depths 0-3, a user-fixed root).
 * ``ref_m2l.npz``: SHA-256 digests of every M2L class matrix at p = 4, 6, 8
   and a few full matrices.
+* ``ref_potentials.npz``: CpuStatic potentials and fields (output 'both',
+  FP64 and FP32 plans) for the N=20,000 / p=4 / depth 4 cloud, an N=4,096,
+  p=8, depth 3 cloud and the separate-target case, plus a cloud with a
+  coincident non-identity pair.
 
 Inputs are stored with the outputs, so the fixtures do not depend on NumPy's
 generator streams.
@@ -128,9 +132,52 @@ def m2l():
     np.savez_compressed(os.path.join(HERE, "ref_m2l.npz"), **out)
 
 
+def potentials():
+    out = {}
+    cases = [("c1", 20000, 4, 4, 314160), ("n4096_p8_d3", 4096, 8, 3, 1083)]
+    for name, n, p, d, seed in cases:
+        x, m = uniform_cloud(n, seed)
+        if name == "c1":
+            m = m / np.linalg.norm(m, axis=1, keepdims=True)
+        ids = np.arange(n, dtype=np.int32)
+        out[f"{name}/positions"] = x
+        out[f"{name}/moments"] = m
+        out[f"{name}/config"] = np.array([n, p, d], dtype=np.int64)
+        for prec in (64, 32):
+            t = time.time()
+            f = O.CpuFmm(x, x, p, d, prec, fixed_ids=ids, kind="ref")
+            r = f.evaluate(m, ids, output="both")
+            out[f"{name}/phi{prec}"] = r["phi"]
+            out[f"{name}/H{prec}"] = r["H"]
+            print(f"potential {name} fp{prec}: {time.time() - t:.1f}s", flush=True)
+    rng = np.random.default_rng(77)
+    s = rng.uniform(-0.9, 0.9, size=(3000, 3))
+    t = rng.uniform(-0.9, 0.9, size=(2000, 3))
+    m = rng.normal(size=(3000, 3))
+    for prec in (64, 32):
+        f = O.CpuFmm(s, t, 4, 3, prec, root_centre=(0.0, 0.0, 0.0), root_half_width=1.0, kind="ref")
+        r = f.evaluate(m, output="both")
+        out[f"separate/phi{prec}"] = r["phi"]
+        out[f"separate/H{prec}"] = r["H"]
+    out["separate/sources"] = s
+    out["separate/targets"] = t
+    out["separate/moments"] = m
+    # Coincident non-identity pair: points 0 and 1 coincide, target 1 has no
+    # identity, so its potential (and field) is NaN.
+    x = np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3], [0.8, 0.7, 0.6], [0.4, 0.5, 0.9]])
+    m = np.eye(4, 3) + 0.1
+    for prec in (64, 32):
+        f = O.CpuFmm(x, x, 2, 1, prec, kind="ref")
+        r = f.evaluate(m, np.array([0, -1, 2, 3], dtype=np.int32), output="both")
+        out[f"coincident/phi{prec}"] = r["phi"]
+    out["coincident/positions"] = x
+    out["coincident/moments"] = m
+    np.savez_compressed(os.path.join(HERE, "ref_potentials.npz"), **out)
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         raise SystemExit("build the reference first: make -C oracle ref")
-    which = sys.argv[1:] or ["trees", "fields", "m2l"]
+    which = sys.argv[1:] or ["trees", "fields", "m2l", "potentials"]
     for w in which:
         globals()[w]()

@@ -278,8 +278,10 @@ def test_identity_map_contract(hip):
     np.testing.assert_array_equal(g.evaluate(m)["H"], g.evaluate(m, target_source_indices=ids)["H"])
     with pytest.raises(ValueError, match="identity map changed"):
         g.evaluate(m, target_source_indices=np.full(500, -1))
-    with pytest.raises(ValueError, match="field-only"):
-        g.evaluate(m, output="both")
+    both = g.evaluate(m, output="both")
+    assert rel_l2(both["H"], g.evaluate(m)["H"]) <= 1e-6 and np.isfinite(both["phi"]).all()
+    with pytest.raises(ValueError, match="output must be"):
+        g.evaluate(m, output="none")
     with pytest.raises(ValueError, match="identity map contains an invalid index"):
         g.evaluate(m[:-1])
     with pytest.raises(ValueError, match="dimensions are inconsistent"):
@@ -324,9 +326,14 @@ def test_empty_geometry(hip):
     for prec in (32, 64):
         f = make(hip, e, e, 3, 2, prec)
         assert f.evaluate(e)["H"].shape == (0, 3)
+        assert f.evaluate(e, output="both")["phi"].shape == (0,)
     x = np.random.default_rng(1).uniform(size=(50, 3))
     f = make(hip, x, e, 3, 2, 32)
     assert f.evaluate(np.ones((50, 3)))["H"].shape == (0, 3)
+    assert f.evaluate(np.ones((50, 3)), output="potential")["phi"].shape == (0,)
+    g = make(hip, e, x, 3, 2, 64)
+    r = g.evaluate(e, output="both")
+    assert not np.any(r["H"]) and not np.any(r["phi"])
 
 
 def test_device_resident_evaluate_matches_host(hip):

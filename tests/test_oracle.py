@@ -145,6 +145,38 @@ def test_separate_targets_match_golden():
         assert rel_l2(f.evaluate(g["separate/moments"])["H"], g[f"separate/H{prec}"]) <= tol
 
 
+@pytest.mark.parametrize("case", ["c1", "n4096_p8_d3", "separate"])
+def test_potentials_match_golden(case):
+    # CpuStatic 'both' output of the compiled reference: potential rows of
+    # the L2P (src/static_operators.cpp:300-335) and the near-field potential
+    # (src/uniform_fmm.cpp:1414-1437 for FP32, src/near_field.cpp:30-80 for
+    # FP64), times W for FP32 plans (src/uniform_fmm.cpp:1448-1451).
+    g = golden("ref_potentials.npz")
+    for prec, tol in ((64, 1e-13), (32, 1e-6)):
+        if case == "separate":
+            f = O.CpuFmm(g["separate/sources"], g["separate/targets"], 4, 3, prec, root_centre=(0, 0, 0),
+                         root_half_width=1.0)
+            r = f.evaluate(g["separate/moments"], output="both")
+        else:
+            x, m = g[f"{case}/positions"], g[f"{case}/moments"]
+            n, p, d = (int(v) for v in g[f"{case}/config"])
+            ids = np.arange(n, dtype=np.int32)
+            r = O.CpuFmm(x, x, p, d, prec, fixed_ids=ids).evaluate(m, ids, output="both")
+        assert rel_l2(r["phi"], g[f"{case}/phi{prec}"]) <= tol, (case, prec)
+        assert rel_l2(r["H"], g[f"{case}/H{prec}"]) <= tol, (case, prec)
+
+
+def test_coincident_potential_is_nan_in_golden_and_oracle():
+    g = golden("ref_potentials.npz")
+    x, m = g["coincident/positions"], g["coincident/moments"]
+    for prec in (64, 32):
+        ref = g[f"coincident/phi{prec}"]
+        assert np.isnan(ref[0]) and np.isnan(ref[1]) and np.isfinite(ref[2:]).all()
+        r = O.CpuFmm(x, x, 2, 1, prec).evaluate(m, np.array([0, -1, 2, 3], dtype=np.int32), output="both")
+        assert np.isnan(r["phi"][:2]).all()
+        np.testing.assert_allclose(r["phi"][2:], ref[2:], rtol=1e-6)
+
+
 def test_depth_zero_equals_direct_sum():
     # test_uniform_fmm.cpp:429-451: with one leaf every pair is list-1 P2P.
     rng = np.random.default_rng(9281)
