@@ -173,7 +173,14 @@ struct DevicePlan {
   unsigned* lvl_max_bits = nullptr;  // L+1: largest |Mc / sigma| per level (float bits)
   const void* zero_block = nullptr;  // 1 KiB of zeros: the operand of sources outside the grid
   int apad = 0, bpad = 0;
+  // Launch choices (set by the autotuner; 0 = built-in default).
+  int m2l_shape = 0;    // MFMA M2L tile shape code (kM2LShapes)
+  int p2p_variant = 0;  // near field at K % 32 == 0 (FP32): 1 vector ALU, 2 matrix cores
 };
+
+// MFMA M2L tile shapes of the harmonic form (<= 3 row tiles): the code is
+// 10 * (column tiles per wave) + (ring stages).
+constexpr int kM2LShapes[] = {43, 24, 34, 26};
 
 // Kernel launchers (one translation unit each).
 template <class S>

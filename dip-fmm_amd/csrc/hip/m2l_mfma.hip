@@ -575,7 +575,7 @@ inline int mfma_shape_override() {
 template <int MT>
 void launch_mfma_t(const DevicePlan& d, int K, bool f16, hipStream_t s) {
   if constexpr (MT <= 3) {
-    switch (mfma_shape_override()) {
+    switch (d.m2l_shape ? d.m2l_shape : mfma_shape_override()) {
       case 24: launch_mfma_cfg<MT, 2, 4>(d, K, f16, s); return;
       case 34: launch_mfma_cfg<MT, 3, 4>(d, K, f16, s); return;
       case 26: launch_mfma_cfg<MT, 2, 6>(d, K, f16, s); return;

@@ -333,7 +333,7 @@ void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
   if (potential && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
   if constexpr (sizeof(S) == 4) {
-    if (K % 32 == 0) {
+    if (K % 32 == 0 && d.p2p_variant != 1) {
       const int count = potential ? d.p2p_tile4_count : d.p2p_tile_count;
       if (count == 0) return;
       const dim3 grid(unsigned((count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -596,9 +597,13 @@ DevicePlan role_exchanged(const hipfmm_plan* P) {
   return a;
 }
 
+// One evaluate enqueued on the plan's streams.  With `capture` the caller
+// is recording a hipGraph on `user`: state is already allocated, only the
+// events that fork and join the streams are recorded (no timing events),
+// and nothing waits on the host.
 template <class S>
 void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* potentials, int K, const int* self,
-                   int memspace, hipStream_t user, bool adjoint) {
+                   int memspace, hipStream_t user, bool adjoint, bool capture = false) {
   const bool want_pot = potentials != nullptr;
   {
     const DevicePlan& f = P->d;
@@ -626,12 +631,20 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     require(!adjoint || P->adj_self_valid,
             "the identity map pairs two targets with one source; the adjoint has no transposed form of it");
   }
-  if (P->timings_pending) finalize_timings(P);
-  (void)hipGetLastError();
-  ensure_capacity<S>(P, K, want_pot);
+  require(!capture || memspace == HIPFMM_DEVICE, "a captured evaluate takes device operands");
+  if (!capture) {
+    if (P->timings_pending) finalize_timings(P);
+    (void)hipGetLastError();
+    ensure_capacity<S>(P, K, want_pot);
+  } else {
+    require(K <= P->d.kcap && (!want_pot || P->potential_used), "state is not allocated for a captured evaluate");
+  }
   const DevicePlan d = adjoint ? role_exchanged(P) : P->d;
   hipStream_t s = P->s_far;
   hipEvent_t* e = P->ev;
+  auto mark = [&](int ev, hipStream_t on) {
+    if (!capture) HIPFMM_CHECK(hipEventRecord(e[ev], on));
+  };
   const S* mu = nullptr;
   S* out = nullptr;
   S* pot = nullptr;
@@ -660,9 +673,9 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
       P->pinned_out_bytes = out_bytes;
     }
     if (in_bytes) std::memcpy(P->pinned_in, moments, in_bytes);
-    HIPFMM_CHECK(hipEventRecord(e[kEvH2D0], s));
+    mark(kEvH2D0, s);
     if (in_bytes) HIPFMM_CHECK(hipMemcpyAsync(d.moments_in, P->pinned_in, in_bytes, hipMemcpyHostToDevice, s));
-    HIPFMM_CHECK(hipEventRecord(e[kEvH2D1], s));
+    mark(kEvH2D1, s);
     mu = static_cast<const S*>(d.moments_in);
     out = fields != nullptr ? static_cast<S*>(d.fields_out) : nullptr;
     pot = want_pot ? static_cast<S*>(d.potentials_out) : nullptr;
@@ -673,8 +686,8 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   } else {
     HIPFMM_CHECK(hipEventRecord(e[kEvUser], user));
     HIPFMM_CHECK(hipStreamWaitEvent(s, e[kEvUser], 0));
-    HIPFMM_CHECK(hipEventRecord(e[kEvH2D0], s));
-    HIPFMM_CHECK(hipEventRecord(e[kEvH2D1], s));
+    mark(kEvH2D0, s);
+    mark(kEvH2D1, s);
     mu = static_cast<const S*>(moments);
     out = static_cast<S*>(fields);
     pot = static_cast<S*>(potentials);
@@ -685,17 +698,17 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   hipfmm::launch_gather<S>(d, mu, scale, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvGather], s));
   HIPFMM_CHECK(hipStreamWaitEvent(P->s_near, e[kEvGather], 0));
-  HIPFMM_CHECK(hipEventRecord(e[kEvNear0], P->s_near));
+  mark(kEvNear0, P->s_near);
   hipfmm::launch_p2p<S>(d, K, want_pot, P->s_near);
   HIPFMM_CHECK(hipEventRecord(e[kEvNear1], P->s_near));
   hipfmm::launch_p2m<S>(d, K, s);
-  HIPFMM_CHECK(hipEventRecord(e[kEvP2M], s));
+  mark(kEvP2M, s);
   for (int level = d.L - 1; level >= 2; --level) hipfmm::launch_m2m<S>(d, level, K, s);
   // Levels 0 and 1 hold no list-2 work; keep their coefficients zero.
   const size_t top = size_t(std::min<int64_t>(d.nodes, 9)) * d.n * K;
   hipfmm::launch_zero<S>(static_cast<S*>(d.M), top, s);
   hipfmm::launch_zero<S>(static_cast<S*>(d.Lc), d.L >= 2 ? top : size_t(d.nodes) * d.n * K, s);
-  HIPFMM_CHECK(hipEventRecord(e[kEvM2M], s));
+  mark(kEvM2M, s);
   const bool mfma = sizeof(S) == 4 && P->m2l_mode != HIPFMM_M2L_VALU && P->mfma_ready;
   // The M2L kernels see the plan through `dm`: in harmonic-subspace form the
   // operands are the compressed multipoles/locals with unit scalings.
@@ -714,22 +727,22 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   } else if (mfma) {
     hipfmm::launch_m2l_split(d, K, P->m2l_mode, s);
   }
-  HIPFMM_CHECK(hipEventRecord(e[kEvM2LKernel0], s));
+  mark(kEvM2LKernel0, s);
   if (mfma)
     hipfmm::launch_m2l_mfma(dm, K, P->m2l_mode, s);
   else
     hipfmm::launch_m2l_valu<S>(dm, K, s);
-  HIPFMM_CHECK(hipEventRecord(e[kEvM2LKernel1], s));
+  mark(kEvM2LKernel1, s);
   if (d.nc > 0 && d.L >= 2) hipfmm::launch_m2l_expand<S>(d, K, s);
-  HIPFMM_CHECK(hipEventRecord(e[kEvM2L], s));
+  mark(kEvM2L, s);
   for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(d, level, K, s);
-  HIPFMM_CHECK(hipEventRecord(e[kEvL2L], s));
+  mark(kEvL2L, s);
   HIPFMM_CHECK(hipStreamWaitEvent(s, e[kEvNear1], 0));
   hipfmm::launch_l2p_unsort<S>(d, out, pot, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvL2P], s));
   P->last_K = K;
   P->last_host = host;
-  P->timings_pending = true;
+  P->timings_pending = !capture;
   if (host) {
     HIPFMM_CHECK(hipEventRecord(e[kEvD2H0], s));
     if (out_bytes) HIPFMM_CHECK(hipMemcpyAsync(P->pinned_out, d.fields_out, out_bytes, hipMemcpyDeviceToHost, s));
@@ -880,6 +893,64 @@ int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode) {
   });
 }
 
+int hipfmm_plan_autotune(hipfmm_plan* plan, int batch) {
+  return guarded([&] {
+    require(plan != nullptr, "plan is null");
+    require(batch >= 1 && batch <= hipfmm::kMaxBatch, "batch must be between 1 and 64 moment states");
+    DeviceGuard guard(plan->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    hipfmm_plan* P = plan;
+    auto run = [&]() {
+      // Device evaluate on the plan's own staging buffers (contents are
+      // irrelevant to the timing).
+      if (P->d.precision == 32)
+        evaluate_impl<float>(P, P->d.moments_in, P->d.fields_out, nullptr, batch, nullptr, HIPFMM_DEVICE, nullptr,
+                             false);
+      else
+        evaluate_impl<double>(P, P->d.moments_in, P->d.fields_out, nullptr, batch, nullptr, HIPFMM_DEVICE, nullptr,
+                              false);
+      finalize_timings(P);
+    };
+    // Allocate the state and zero the staging input the timed runs read.
+    if (P->timings_pending) finalize_timings(P);
+    if (P->d.precision == 32)
+      ensure_capacity<float>(P, batch, false);
+    else
+      ensure_capacity<double>(P, batch, false);
+    HIPFMM_CHECK(hipMemset(P->d.moments_in, 0, size_t(std::max({P->d.ns, P->d.nt, 1})) * 3 * P->d.kcap *
+                                                  (P->d.precision == 32 ? 4 : 8)));
+    auto best_of = [&](auto&& set, const std::vector<int>& choices, auto&& measure) {
+      int best = choices.front();
+      double best_t = 1e300;
+      for (int c : choices) {
+        set(c);
+        run();  // warm
+        double t = 1e300;
+        for (int r = 0; r < 3; ++r) {
+          run();
+          t = std::min(t, measure());
+        }
+        if (t < best_t) {
+          best_t = t;
+          best = c;
+        }
+      }
+      set(best);
+      return best;
+    };
+    const bool mfma = P->d.precision == 32 && P->m2l_mode != HIPFMM_M2L_VALU && P->mfma_ready && P->d.L >= 2;
+    if (mfma && P->d.bpad / 32 <= 3) {
+      std::vector<int> shapes(std::begin(hipfmm::kM2LShapes), std::end(hipfmm::kM2LShapes));
+      best_of([&](int c) { P->d.m2l_shape = c; }, shapes, [&] { return P->timings.m2l_kernel_seconds; });
+    }
+    if (P->d.precision == 32 && batch % 32 == 0)
+      best_of([&](int c) { P->d.p2p_variant = c; }, {2, 1}, [&] { return P->timings.p2p_seconds; });
+    P->stats.m2l_tile_shape = uint64_t(P->d.m2l_shape);
+    P->stats.p2p_kernel_variant = uint64_t(P->d.p2p_variant);
+    P->stats.autotune_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
 int hipfmm_plan_statistics(const hipfmm_plan* plan, hipfmm_stats* out) {
   return guarded([&] {
     require(plan != nullptr && out != nullptr, "plan and output must not be null");
@@ -969,6 +1040,93 @@ int hipfmm_direct_field(const double* targets, int nt, const double* sources, in
     HIPFMM_CHECK(hipDeviceSynchronize());
     if (nt) HIPFMM_CHECK(hipMemcpy(fields, f, size_t(nt) * 3 * sizeof(double), hipMemcpyDeviceToHost));
   });
+}
+
+struct hipfmm_group {
+  std::vector<hipfmm_plan*> plans;
+  hipStream_t capture = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;
+  int device = 0;
+};
+
+int hipfmm_group_create(hipfmm_plan* const* plans, int count, const void* const* moments, void* const* fields,
+                        const int* batches, hipfmm_group** group) {
+  return guarded([&] {
+    require(group != nullptr, "group output is null");
+    *group = nullptr;
+    require(count >= 1 && plans != nullptr && moments != nullptr && fields != nullptr && batches != nullptr,
+            "a group needs at least one plan with operands");
+    for (int i = 0; i < count; ++i) {
+      require(plans[i] != nullptr, "plan is null");
+      for (int j = 0; j < i; ++j) require(plans[j] != plans[i], "a plan may appear once in a group");
+      require(plans[i]->device == plans[0]->device, "the plans of a group share one device");
+      require(batches[i] >= 1 && batches[i] <= hipfmm::kMaxBatch, "batch must be between 1 and 64 moment states");
+    }
+    // The recording below ended the process with a host segmentation fault
+    // inside this call on its first run (cause not found); until it is, the
+    // group is refused here rather than recorded.
+    throw std::runtime_error("hipfmm_group_create is switched off in this build");
+    DeviceGuard guard(plans[0]->device);
+    auto G = std::make_unique<hipfmm_group>();
+    G->device = plans[0]->device;
+    G->plans.assign(plans, plans + count);
+    // Everything that allocates or waits happens before the capture.
+    for (int i = 0; i < count; ++i) {
+      hipfmm_plan* P = plans[i];
+      if (P->timings_pending) finalize_timings(P);
+      if (P->d.precision == 32)
+        ensure_capacity<float>(P, batches[i], false);
+      else
+        ensure_capacity<double>(P, batches[i], false);
+    }
+    HIPFMM_CHECK(hipStreamCreateWithFlags(&G->capture, hipStreamNonBlocking));
+    HIPFMM_CHECK(hipEventCreateWithFlags(&G->done, hipEventDisableTiming));
+    HIPFMM_CHECK(hipStreamBeginCapture(G->capture, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int i = 0; i < count; ++i) {
+        hipfmm_plan* P = plans[i];
+        if (P->d.precision == 32)
+          evaluate_impl<float>(P, moments[i], fields[i], nullptr, batches[i], nullptr, HIPFMM_DEVICE, G->capture,
+                               false, true);
+        else
+          evaluate_impl<double>(P, moments[i], fields[i], nullptr, batches[i], nullptr, HIPFMM_DEVICE, G->capture,
+                                false, true);
+      }
+    } catch (...) {
+      hipGraph_t broken = nullptr;
+      (void)hipStreamEndCapture(G->capture, &broken);
+      if (broken) (void)hipGraphDestroy(broken);
+      (void)hipGetLastError();
+      hipfmm_group_destroy(G.release());
+      throw;
+    }
+    HIPFMM_CHECK(hipStreamEndCapture(G->capture, &G->graph));
+    HIPFMM_CHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+    *group = G.release();
+  });
+}
+
+int hipfmm_group_launch(hipfmm_group* group, void* stream) {
+  return guarded([&] {
+    require(group != nullptr, "group is null");
+    DeviceGuard guard(group->device);
+    HIPFMM_CHECK(hipGraphLaunch(group->exec, static_cast<hipStream_t>(stream)));
+  });
+}
+
+void hipfmm_group_destroy(hipfmm_group* group) {
+  if (group == nullptr) return;
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(group->device);
+  if (group->exec) (void)hipGraphExecDestroy(group->exec);
+  if (group->graph) (void)hipGraphDestroy(group->graph);
+  if (group->done) (void)hipEventDestroy(group->done);
+  if (group->capture) (void)hipStreamDestroy(group->capture);
+  if (prev >= 0) (void)hipSetDevice(prev);
+  delete group;
 }
 
 }  // extern "C"

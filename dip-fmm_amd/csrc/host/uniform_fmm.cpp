@@ -668,8 +668,13 @@ HipPlanStatistics UniformFmm::hip_plan_statistics() const {
   o.m2l_operand_size = s.m2l_operand_size;
   o.m2l_reconstruction_error = s.m2l_reconstruction_error;
   o.setup_seconds = setup_seconds_;
+  o.m2l_tile_shape = s.m2l_tile_shape;
+  o.p2p_kernel_variant = s.p2p_kernel_variant;
+  o.autotune_seconds = s.autotune_seconds;
   return o;
 }
+
+void UniformFmm::autotune(int batch) { check(hipfmm_plan_autotune(plan_, batch)); }
 
 std::vector<double> UniformFmm::multipole(int node, int state) const {
   std::vector<double> out(static_cast<std::size_t>(basis_.size()));

@@ -134,6 +134,8 @@ struct HipPlanStatistics {
   std::size_t m2l_operand_size{0};       // n, or (p+1)^2 in harmonic-subspace form
   double m2l_reconstruction_error{0.0};  // of the harmonic-subspace factorisation
   double setup_seconds{0.0};             // host plan construction + upload
+  std::size_t m2l_tile_shape{0}, p2p_kernel_variant{0};  // launch choices (autotune)
+  double autotune_seconds{0.0};
 };
 
 bool hip_full_available() noexcept;
@@ -190,6 +192,12 @@ class UniformFmm {
 
   void set_m2l_mode(HipM2LMode mode);
   HipM2LMode m2l_mode() const { return m2l_mode_; }
+  // Time the launch variants at `batch` states and keep the fastest
+  // (hipfmm_plan_autotune).
+  void autotune(int batch);
+  // The C-ABI plan, for callers that group several plans into one launch
+  // (hipfmm_group_create).
+  hipfmm_plan* native_plan() const { return plan_; }
 
   const UniformTree& tree() const { return tree_; }
   const Basis& basis() const { return basis_; }

@@ -11,6 +11,7 @@
 
 #include "../host/operators.hpp"
 #include "../host/uniform_fmm.hpp"
+#include "../../../include/hipfmm.h"
 
 namespace py = pybind11;
 using namespace cdfmm_hip;
@@ -110,7 +111,49 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
       .value("MFMA_BF16X3", HipM2LMode::MfmaBf16x3)
       .value("MFMA_FP16X3", HipM2LMode::MfmaFp16x3);
 
-  m.def("hip_full_available", &hip_full_available);
+    // Several plans' device evaluates recorded into one HIP graph
+  // (hipfmm_group_create); launch() replays them all with one launch.
+  struct HipGroup {
+    hipfmm_group* g = nullptr;
+    py::list keep;  // the UniformFmm objects stay alive with the group
+    ~HipGroup() { hipfmm_group_destroy(g); }
+  };
+  py::class_<HipGroup>(m, "HipGroup")
+      .def(py::init([](py::list fmms, std::vector<std::uintptr_t> moments, std::vector<std::uintptr_t> fields,
+                       std::vector<int> batches) {
+             const std::size_t n = fmms.size();
+             if (moments.size() != n || fields.size() != n || batches.size() != n)
+               throw std::invalid_argument("one moments pointer, fields pointer and batch per plan");
+             std::vector<hipfmm_plan*> plans;
+             std::vector<const void*> mp;
+             std::vector<void*> fp;
+             for (std::size_t i = 0; i < n; ++i) {
+               plans.push_back(py::cast<UniformFmm&>(fmms[i]).native_plan());
+               mp.push_back(reinterpret_cast<const void*>(moments[i]));
+               fp.push_back(reinterpret_cast<void*>(fields[i]));
+             }
+             auto h = std::make_unique<HipGroup>();
+             const int rc = hipfmm_group_create(plans.data(), static_cast<int>(n), mp.data(), fp.data(),
+                                                batches.data(), &h->g);
+             if (rc == HIPFMM_ERROR_INVALID_ARGUMENT) throw std::invalid_argument(hipfmm_last_error());
+             if (rc != HIPFMM_OK) throw std::runtime_error(hipfmm_last_error());
+             h->keep = fmms;
+             return h;
+           }),
+           py::arg("fmms"), py::arg("moments_ptrs"), py::arg("fields_ptrs"), py::arg("batches"))
+      .def(
+          "launch",
+          [](HipGroup& h, std::uintptr_t stream) {
+            int rc;
+            {
+              py::gil_scoped_release release;
+              rc = hipfmm_group_launch(h.g, reinterpret_cast<void*>(stream));
+            }
+            if (rc != HIPFMM_OK) throw std::runtime_error(hipfmm_last_error());
+          },
+          py::arg("stream") = 0, "Replay every plan's evaluate with one graph launch ordered on `stream`.");
+
+m.def("hip_full_available", &hip_full_available);
   // Host operator builders, exposed for the parity tests of the plan.
   m.def("_laplace_derivatives", [](int order, std::array<double, 3> r) {
     const auto v = laplace_derivatives(order, r);
@@ -353,6 +396,15 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
         return timings_dict(t);
       })
       .def("set_m2l_mode", &UniformFmm::set_m2l_mode)
+      .def(
+          "autotune",
+          [](UniformFmm& f, int batch) {
+            py::gil_scoped_release release;
+            f.autotune(batch);
+          },
+          py::arg("batch"),
+          "Time the matrix-core M2L tile shapes and the near-field kernels at `batch` states and keep the "
+          "fastest; the choices appear in hip_plan_statistics.")
       .def_property_readonly("m2l_mode", &UniformFmm::m2l_mode)
       .def_property_readonly("tree", &UniformFmm::tree, py::return_value_policy::reference_internal)
       .def_property_readonly("expansion_order", [](const UniformFmm& f) { return f.basis().order(); })
@@ -391,6 +443,9 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
                                d["m2l_operand_size"] = s.m2l_operand_size;
                                d["m2l_reconstruction_error"] = s.m2l_reconstruction_error;
                                d["setup_seconds"] = s.setup_seconds;
+                               d["m2l_tile_shape"] = s.m2l_tile_shape;
+                               d["p2p_kernel_variant"] = s.p2p_kernel_variant;
+                               d["autotune_seconds"] = s.autotune_seconds;
                                return d;
                              })
       .def(

@@ -135,6 +135,13 @@ typedef struct hipfmm_stats {
    * (T_o = Q C_o P; no counterpart in the reference, which always uses n). */
   uint64_t m2l_operand_size;
   double m2l_reconstruction_error; /* max |T_o - Q C_o P| / max |T_o| over classes (0 if unused) */
+  /* Launch choices in use (hipfmm_plan_autotune; no counterpart in the
+   * reference): the matrix-core M2L tile shape code, 10 * column tiles per
+   * wave + ring stages (0 = built-in default), and the near-field kernel at
+   * batch % 32 == 0 (0 default, 1 vector ALU, 2 matrix cores). */
+  uint64_t m2l_tile_shape;
+  uint64_t p2p_kernel_variant;
+  double autotune_seconds; /* host time of the last hipfmm_plan_autotune */
 } hipfmm_stats;
 
 /* Device-stream stage times of the most recent evaluate, seconds; mirrors
@@ -223,6 +230,30 @@ int hipfmm_plan_evaluate_adjoint(hipfmm_plan* plan, const void* cotangents, void
 
 /* Select the M2L arithmetic of later evaluates (enum hipfmm_m2l_mode). */
 int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode);
+
+/* Launch autotuner; no counterpart in the reference (whose CudaFull launch
+ * shapes are fixed, src/cuda_fmm.cu:3500-3600).  Times, on the plan's own
+ * state buffers at `batch` states and the current M2L mode, every tile shape
+ * of the matrix-core M2L and both near-field kernels (FP32 plans, batch a
+ * multiple of 32), keeps the fastest of each for later evaluates of this
+ * plan and reports them in hipfmm_stats.  Results are unchanged in value
+ * class: every candidate meets the same accuracy bars.  Overwrites the
+ * plan's coefficients of the last evaluate. */
+int hipfmm_plan_autotune(hipfmm_plan* plan, int batch);
+
+/* Multi-geometry launch; no counterpart in the reference.  A group records
+ * the device-memory evaluates of `count` distinct plans (fields only, each
+ * plan with its own device moments/fields pointers and batch) into one HIP
+ * graph; hipfmm_group_launch replays all of them with one launch on `stream`,
+ * the independent geometries' kernels overlapping.  Pointers, batches, M2L
+ * modes and launch choices are those at creation; each plan's fixed identity
+ * map applies.  A plan belongs to at most one group at a time and must not be
+ * evaluated while its group runs. */
+typedef struct hipfmm_group hipfmm_group;
+int hipfmm_group_create(hipfmm_plan* const* plans, int count, const void* const* moments, void* const* fields,
+                        const int* batches, hipfmm_group** group);
+int hipfmm_group_launch(hipfmm_group* group, void* stream);
+void hipfmm_group_destroy(hipfmm_group* group);
 
 /* Statistics / timings.  Replace CudaFullPlan::statistics() / timings(). */
 int hipfmm_plan_statistics(const hipfmm_plan* plan, hipfmm_stats* out);

@@ -1,0 +1,154 @@
+"""Launch machinery of the HipFull path (needs one MI355X): the autotuner and
+the multi-geometry group launch recorded as one HIP graph.
+
+Neither has a counterpart in the reference.  The bar is the path's own: any
+launch choice meets the FP32 bar against the FP64 FMM, and a graph replay
+reproduces the eager evaluate of the same plan bit for bit, on every replay,
+with eager work and synchronisations between the replays.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import cdfmm_hip
+
+    assert cdfmm_hip.hip_full_available(), "no gfx950 device is visible to the HIP runtime"
+    return cdfmm_hip
+
+
+def make(hip, src, tgt, p, depth, prec, ids=None, mode=None):
+    o = hip.UniformFmmOptions()
+    o.expansion_order = p
+    o.tree.max_level = depth
+    o.backend = hip.ExecutionBackend.HIP_FULL
+    o.precision = hip.StaticPrecision.FLOAT32 if prec == 32 else hip.StaticPrecision.FLOAT64
+    if ids is not None:
+        o.fixed_target_source_indices = [int(v) for v in ids]
+    if mode is not None:
+        o.hip_m2l_mode = mode
+    return hip.UniformFmm(src, tgt, o)
+
+
+def announce(*what):
+    print("case:", *what, flush=True)
+
+
+SHAPES = {43, 24, 34, 26}
+
+
+@pytest.mark.parametrize("mode", ["MFMA_BF16X3", "MFMA_FP16X3"])
+def test_autotune_choices_meet_the_bar(hip, mode):
+    rng = np.random.default_rng(950)
+    n, K = 12000, 32
+    x = rng.uniform(0, 1, size=(n, 3))
+    ms = rng.normal(size=(K, n, 3))
+    announce("autotune", mode)
+    f = make(hip, x, x, 6, 3, 32, ids=np.arange(n), mode=getattr(hip.HipM2LMode, mode))
+    before = f.evaluate_batch(ms)
+    f.autotune(K)
+    st = f.hip_plan_statistics
+    print(f"  shape {st['m2l_tile_shape']} near-field variant {st['p2p_kernel_variant']} "
+          f"in {st['autotune_seconds']:.2f}s", flush=True)
+    assert st["m2l_tile_shape"] in SHAPES and st["p2p_kernel_variant"] in (1, 2)
+    after = f.evaluate_batch(ms)
+    ref = O.CpuFmm(x, x, 6, 3, 64, fixed_ids=np.arange(n))
+    for k in (0, K - 1):
+        r = ref.evaluate(ms[k])["H"]
+        assert rel_l2(after[k], r) <= 1e-5
+        assert rel_l2(after[k], before[k]) <= 2e-6
+
+
+def test_autotune_on_vector_paths(hip):
+    # FP64 plan, odd batch: nothing to choose; the plan keeps its defaults.
+    rng = np.random.default_rng(951)
+    x = rng.uniform(0, 1, size=(3000, 3))
+    f = make(hip, x, x, 4, 3, 64, ids=np.arange(3000))
+    f.autotune(3)
+    st = f.hip_plan_statistics
+    assert st["m2l_tile_shape"] == 0 and st["p2p_kernel_variant"] == 0
+
+
+@pytest.mark.skip(reason="hipfmm_group_create is switched off: its first GPU run ended the process with a host "
+                         "segmentation fault inside the stream-capture recording; cause not found")
+def test_group_graph_replays_match_eager(hip):
+    import torch
+
+    rng = np.random.default_rng(960)
+    specs = [(3000, 4, 3, 32, 8, None), (20000, 6, 4, 32, 32, "MFMA_BF16X3"), (5000, 4, 3, 64, 3, None),
+             (8000, 8, 3, 32, 32, "MFMA_FP16X3")]
+    fmms, moms, outs, eager, pts = [], [], [], [], []
+    stream = torch.cuda.current_stream().cuda_stream
+    for n, p, d, prec, K, mode in specs:
+        x = rng.uniform(0, 1, size=(n, 3))
+        f = make(hip, x, x, p, d, prec, ids=np.arange(n), mode=None if mode is None else getattr(hip.HipM2LMode, mode))
+        dt = torch.float32 if prec == 32 else torch.float64
+        m = torch.tensor(rng.normal(size=(K, n, 3)), dtype=dt, device="cuda")
+        fmms.append(f)
+        pts.append(x)
+        moms.append(m)
+        outs.append(torch.full((K, n, 3), float("nan"), dtype=dt, device="cuda"))
+        eager.append(torch.empty((K, n, 3), dtype=dt, device="cuda"))
+    announce("group graph", len(specs))
+
+    def run_eager():
+        for f, m, e in zip(fmms, moms, eager):
+            f.evaluate_device(m.data_ptr(), e.data_ptr(), m.shape[0], stream)
+        torch.cuda.synchronize()
+
+    group = hip.HipGroup(fmms, [m.data_ptr() for m in moms], [o.data_ptr() for o in outs],
+                         [m.shape[0] for m in moms])
+    for replay in range(3):
+        if replay == 2:
+            for m in moms:  # the graph reads the moments where they live
+                m.mul_(-0.5)
+        run_eager()
+        junk = torch.randn(1 << 20, device="cuda").sum()  # unrelated eager work between replays
+        for o in outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
+        group.launch(stream)
+        torch.cuda.synchronize()
+        for i, (o, e) in enumerate(zip(outs, eager)):
+            np.testing.assert_array_equal(o.cpu().numpy(), e.cpu().numpy(), err_msg=f"replay {replay} plan {i}")
+        print(f"  replay {replay}: all {len(specs)} geometries bitwise equal to eager ({float(junk):.1f})",
+              flush=True)
+    # The FP64 geometry against the oracle, as a value check of the replay.
+    n, p, d = specs[2][:3]
+    ref = O.CpuFmm(pts[2], pts[2], p, d, 64, fixed_ids=np.arange(n))
+    m0 = moms[2][0].cpu().numpy()
+    assert rel_l2(outs[2][0].cpu().numpy(), ref.evaluate(m0)["H"]) <= 1e-12
+
+
+def test_group_is_refused_while_switched_off(hip):
+    import torch
+
+    rng = np.random.default_rng(971)
+    x = rng.uniform(0, 1, size=(1000, 3))
+    f = make(hip, x, x, 3, 2, 32)
+    m = torch.zeros((4, 1000, 3), device="cuda")
+    h = torch.zeros((4, 1000, 3), device="cuda")
+    with pytest.raises(RuntimeError, match="switched off"):
+        hip.HipGroup([f], [m.data_ptr()], [h.data_ptr()], [4])
+
+
+def test_group_rejects_duplicates_and_bad_batches(hip):
+    import torch
+
+    rng = np.random.default_rng(970)
+    x = rng.uniform(0, 1, size=(1000, 3))
+    f = make(hip, x, x, 3, 2, 32)
+    m = torch.zeros((4, 1000, 3), device="cuda")
+    h = torch.zeros((4, 1000, 3), device="cuda")
+    with pytest.raises(ValueError, match="once in a group"):
+        hip.HipGroup([f, f], [m.data_ptr()] * 2, [h.data_ptr()] * 2, [4, 4])
+    with pytest.raises(ValueError, match="batch"):
+        hip.HipGroup([f], [m.data_ptr()], [h.data_ptr()], [65])
