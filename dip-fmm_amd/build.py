@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 HIP_SOURCES = ["plan.hip", "far.hip", "near.hip", "m2l_valu.hip", "m2l_mfma.hip", "m2l_basis.hip"]
-HOST_SOURCES = ["host/operators.cpp", "host/uniform_fmm.cpp", "python/bindings.cpp"]
+HOST_SOURCES = ["host/operators.cpp", "host/operator_tables.cpp", "host/uniform_fmm.cpp", "python/bindings.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

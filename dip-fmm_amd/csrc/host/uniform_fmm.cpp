@@ -8,6 +8,7 @@
 #include <stdexcept>
 
 #include "../../../include/hipfmm.h"
+#include "operator_tables.hpp"
 #include "operators.hpp"
 
 namespace cdfmm_hip {
@@ -251,6 +252,7 @@ UniformFmm::UniformFmm(const std::vector<Vec3>& sources, const std::vector<Vec3>
     fixed_ids_ = ids;
   }
   m2l_mode_ = options.hip_m2l_mode;
+  operator_cache_ = options.operator_cache;
   build_plan(options);
 }
 
@@ -267,7 +269,9 @@ UniformFmm::UniformFmm(UniformFmm&& other) noexcept
       plan_(other.plan_),
       setup_seconds_(other.setup_seconds_),
       last_timings_(other.last_timings_),
-      aggregate_timings_(other.aggregate_timings_) {
+      aggregate_timings_(other.aggregate_timings_),
+      operator_cache_(std::move(other.operator_cache_)),
+      operator_tables_loaded_(other.operator_tables_loaded_) {
   other.plan_ = nullptr;
 }
 
@@ -285,6 +289,8 @@ UniformFmm& UniformFmm::operator=(UniformFmm&& other) noexcept {
     setup_seconds_ = other.setup_seconds_;
     last_timings_ = other.last_timings_;
     aggregate_timings_ = other.aggregate_timings_;
+    operator_cache_ = std::move(other.operator_cache_);
+    operator_tables_loaded_ = other.operator_tables_loaded_;
     other.plan_ = nullptr;
   }
   return *this;
@@ -318,44 +324,21 @@ void UniformFmm::build_plan(const UniformFmmOptions&) {
   for (std::size_t i = 0; i < ss.size(); ++i) frame(ss[i], &src[3 * i]);
   for (std::size_t i = 0; i < ts.size(); ++i) frame(ts[i], &tgt[3 * i]);
   for (int64_t k = 0; k < nodes; ++k) frame(tree_.node_centre(k), &centres[3 * static_cast<std::size_t>(k)]);
-  // M2L classes: every list-2 offset, matrices in FP64.
-  std::vector<std::array<int, 3>> offsets;
-  if (L >= 2) offsets = list2_offsets();
-  std::vector<int> class_offsets(3 * offsets.size());
-  std::vector<double> matrices(offsets.size() * static_cast<std::size_t>(n) * n);
-#pragma omp parallel for schedule(dynamic, 4)
-  for (std::ptrdiff_t k = 0; k < static_cast<std::ptrdiff_t>(offsets.size()); ++k) {
-    const auto m = m2l_class_matrix(basis_, offsets[static_cast<std::size_t>(k)]);
-    std::copy(m.begin(), m.end(), matrices.begin() + k * static_cast<std::ptrdiff_t>(m.size()));
+  // Translation tables: from the operator cache when it holds tables for
+  // this order, depth and frame, else built (and written to the cache).
+  const double frame_width = fp32 ? 1.0 : 2.0 * tree_.root_half_width();
+  std::optional<OperatorTables> tables;
+  operator_tables_loaded_ = false;
+  if (!operator_cache_.empty()) {
+    tables = load_operator_tables(operator_cache_, basis_.order(), L, frame_width, nullptr);
+    operator_tables_loaded_ = tables.has_value();
   }
-  for (std::size_t k = 0; k < offsets.size(); ++k)
-    for (int a = 0; a < 3; ++a) class_offsets[3 * k + a] = offsets[k][a];
-  // Level scalings (src/uniform_fmm.cpp:529-566).
-  std::vector<double> ms(static_cast<std::size_t>(L + 1) * n), ls(ms.size());
-  std::vector<double> iwp(static_cast<std::size_t>(basis_.order() + 2), 1.0);
-  for (int level = 0; level <= L; ++level) {
-    const double physical = 2.0 * (tree_.root_half_width() / static_cast<double>(1 << level));
-    const double w = fp32 ? physical / scale_ : physical;
-    for (int d = 1; d <= basis_.order() + 1; ++d) iwp[d] = iwp[d - 1] / w;
-    for (int i = 0; i < n; ++i) {
-      ms[static_cast<std::size_t>(level) * n + i] = iwp[degree(basis_[i])];
-      ls[static_cast<std::size_t>(level) * n + i] = iwp[degree(basis_[i]) + 1];
-    }
+  if (!tables) {
+    tables = build_operator_tables(basis_, L, frame_width);
+    if (!operator_cache_.empty()) save_operator_tables(*tables, operator_cache_);
   }
-  // Shift operators per level and child class (src/uniform_fmm.cpp:457-500).
-  std::vector<double> m2m(static_cast<std::size_t>(L) * 8 * n * n), l2l(m2m.size());
-  for (int level = 1; level <= L; ++level) {
-    const double physical = tree_.root_half_width() / static_cast<double>(1 << level);
-    const double h = fp32 ? physical / scale_ : physical;
-    for (int cls = 0; cls < 8; ++cls) {
-      const std::array<double, 3> off{(cls & 1) ? h : -h, (cls & 2) ? h : -h, (cls & 4) ? h : -h};
-      const auto up = m2m_dense(basis_, {-off[0], -off[1], -off[2]});
-      const auto down = l2l_dense(basis_, off);
-      const std::size_t at = (static_cast<std::size_t>(level - 1) * 8 + cls) * n * n;
-      std::copy(up.begin(), up.end(), m2m.begin() + static_cast<std::ptrdiff_t>(at));
-      std::copy(down.begin(), down.end(), l2l.begin() + static_cast<std::ptrdiff_t>(at));
-    }
-  }
+  const OperatorTables& op = *tables;
+  const int class_count = static_cast<int>(op.class_offsets.size() / 3);
   std::vector<int> fixed_self;
   if (fixed_ids_.has_value()) fixed_self = sorted_self_indices(fixed_ids_.value());
   hipfmm_plan_desc desc{};
@@ -370,13 +353,13 @@ void UniformFmm::build_plan(const UniformFmmOptions&) {
   desc.target_permutation = tree_.target_permutation().data();
   desc.node_ranges = tree_.node_ranges().data();
   desc.node_centres = centres.data();
-  desc.m2l_class_count = static_cast<int>(offsets.size());
-  desc.m2l_class_offsets = class_offsets.data();
-  desc.m2l_matrices = matrices.data();
-  desc.m2l_multipole_scaling = ms.data();
-  desc.m2l_local_scaling = ls.data();
-  desc.m2m_operators = m2m.data();
-  desc.l2l_operators = l2l.data();
+  desc.m2l_class_count = class_count;
+  desc.m2l_class_offsets = op.class_offsets.data();
+  desc.m2l_matrices = op.m2l.data();
+  desc.m2l_multipole_scaling = op.ms.data();
+  desc.m2l_local_scaling = op.ls.data();
+  desc.m2m_operators = op.m2m.data();
+  desc.l2l_operators = op.l2l.data();
   desc.fixed_self_indices = fixed_self.empty() ? nullptr : fixed_self.data();
   desc.device_moment_scale = fp32 ? 1.0 / (scale_ * scale_ * scale_) : 1.0;
   desc.potential_scale = fp32 ? scale_ : 1.0;
@@ -671,6 +654,7 @@ HipPlanStatistics UniformFmm::hip_plan_statistics() const {
   o.m2l_tile_shape = s.m2l_tile_shape;
   o.p2p_kernel_variant = s.p2p_kernel_variant;
   o.autotune_seconds = s.autotune_seconds;
+  o.operator_tables_loaded = operator_tables_loaded_;
   return o;
 }
 

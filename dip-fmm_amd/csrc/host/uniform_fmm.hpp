@@ -78,6 +78,11 @@ struct UniformFmmOptions {
   ExecutionBackend backend{ExecutionBackend::Auto};
   std::optional<std::vector<int>> fixed_target_source_indices{};
   HipM2LMode hip_m2l_mode{HipM2LMode::Valu};
+  // Translation-table cache file (no counterpart in the reference): when it
+  // holds tables for this order, depth and operator frame they are loaded
+  // instead of built; otherwise they are built and written there.  Empty:
+  // always build.
+  std::string operator_cache{};
 };
 
 // Complete uniform octree with level-wise Morton ordering; the same geometry
@@ -136,6 +141,7 @@ struct HipPlanStatistics {
   double setup_seconds{0.0};             // host plan construction + upload
   std::size_t m2l_tile_shape{0}, p2p_kernel_variant{0};  // launch choices (autotune)
   double autotune_seconds{0.0};
+  bool operator_tables_loaded{false};  // translation tables came from the operator cache
 };
 
 bool hip_full_available() noexcept;
@@ -233,6 +239,8 @@ class UniformFmm {
   hipfmm_plan* plan_{nullptr};
   double setup_seconds_{0.0};
   EvaluationTimings last_timings_{}, aggregate_timings_{};
+  std::string operator_cache_;
+  bool operator_tables_loaded_{false};
 };
 
 }  // namespace cdfmm_hip

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "../host/operator_tables.hpp"
 #include "../host/operators.hpp"
 #include "../host/uniform_fmm.hpp"
 #include "../../../include/hipfmm.h"
@@ -153,6 +154,43 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
           },
           py::arg("stream") = 0, "Replay every plan's evaluate with one graph launch ordered on `stream`.");
 
+  // Translation-table blobs (host only, no device needed).
+  auto tables_dict = [](const OperatorTables& t) {
+    py::dict d;
+    d["order"] = t.order;
+    d["max_level"] = t.max_level;
+    d["frame_root_width"] = t.frame_root_width;
+    d["class_offsets"] = py::array_t<int>(static_cast<py::ssize_t>(t.class_offsets.size()), t.class_offsets.data());
+    auto arr = [](const std::vector<double>& v) { return py::array_t<double>(static_cast<py::ssize_t>(v.size()), v.data()); };
+    d["m2l"] = arr(t.m2l);
+    d["ms"] = arr(t.ms);
+    d["ls"] = arr(t.ls);
+    d["m2m"] = arr(t.m2m);
+    d["l2l"] = arr(t.l2l);
+    return d;
+  };
+  m.def(
+      "build_operator_tables",
+      [tables_dict](int order, int max_level, double frame_root_width) {
+        return tables_dict(build_operator_tables(Basis(order), max_level, frame_root_width));
+      },
+      py::arg("order"), py::arg("max_level"), py::arg("frame_root_width"));
+  m.def(
+      "save_operator_tables",
+      [](const std::string& path, int order, int max_level, double frame_root_width) {
+        save_operator_tables(build_operator_tables(Basis(order), max_level, frame_root_width), path);
+      },
+      py::arg("path"), py::arg("order"), py::arg("max_level"), py::arg("frame_root_width"));
+  m.def(
+      "load_operator_tables",
+      [tables_dict](const std::string& path, int order, int max_level, double frame_root_width) {
+        std::string why;
+        const auto t = load_operator_tables(path, order, max_level, frame_root_width, &why);
+        if (!t) throw std::invalid_argument("operator tables not loaded: " + why);
+        return tables_dict(*t);
+      },
+      py::arg("path"), py::arg("order"), py::arg("max_level"), py::arg("frame_root_width"));
+
 m.def("hip_full_available", &hip_full_available);
   // Host operator builders, exposed for the parity tests of the plan.
   m.def("_laplace_derivatives", [](int order, std::array<double, 3> r) {
@@ -212,6 +250,7 @@ m.def("hip_full_available", &hip_full_available);
       .def_readwrite("precision", &UniformFmmOptions::precision)
       .def_readwrite("expansion_order", &UniformFmmOptions::expansion_order)
       .def_readwrite("tree", &UniformFmmOptions::tree)
+      .def_readwrite("operator_cache", &UniformFmmOptions::operator_cache)
       .def_readwrite("backend", &UniformFmmOptions::backend)
       .def_readwrite("fixed_target_source_indices", &UniformFmmOptions::fixed_target_source_indices)
       .def_readwrite("hip_m2l_mode", &UniformFmmOptions::hip_m2l_mode);
@@ -446,6 +485,7 @@ m.def("hip_full_available", &hip_full_available);
                                d["m2l_tile_shape"] = s.m2l_tile_shape;
                                d["p2p_kernel_variant"] = s.p2p_kernel_variant;
                                d["autotune_seconds"] = s.autotune_seconds;
+                               d["operator_tables_loaded"] = s.operator_tables_loaded;
                                return d;
                              })
       .def(

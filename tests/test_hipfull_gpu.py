@@ -373,3 +373,34 @@ def test_gpu_direct_sum_matches_oracle():
                                  ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), out.ctypes.data_as(dp))
     assert rc == 0
     assert rel_l2(out, O.direct(t, s, m, ids)["H"]) <= 1e-13
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_operator_cache_plans_match_built_plans(hip, prec, tmp_path):
+    # A plan whose translation tables come from the cache file evaluates
+    # bit for bit like one that built them; the first plan writes the file.
+    rng = np.random.default_rng(990 + prec)
+    x = rng.uniform(0, 1, size=(5000, 3))
+    m = rng.normal(size=(5000, 3))
+    ids = np.arange(5000)
+    cache = str(tmp_path / f"tables_{prec}.bin")
+    announce("operator cache", prec)
+
+    def plan(with_cache):
+        o = hip.UniformFmmOptions()
+        o.expansion_order = 6
+        o.tree.max_level = 3
+        o.backend = hip.ExecutionBackend.HIP_FULL
+        o.precision = hip.StaticPrecision.FLOAT32 if prec == 32 else hip.StaticPrecision.FLOAT64
+        o.fixed_target_source_indices = [int(v) for v in ids]
+        if with_cache:
+            o.operator_cache = cache
+        return hip.UniformFmm(x, x, o)
+
+    plain = plan(False).evaluate(m)["H"]
+    first = plan(True)
+    assert not first.hip_plan_statistics["operator_tables_loaded"]
+    second = plan(True)
+    assert second.hip_plan_statistics["operator_tables_loaded"]
+    np.testing.assert_array_equal(first.evaluate(m)["H"], plain)
+    np.testing.assert_array_equal(second.evaluate(m)["H"], plain)

@@ -146,3 +146,60 @@ def test_front_end_errors_follow_the_reference(hip):
         opts.tree.root_centre = hip.Vec3(0.0, 0.0, 0.0)
         opts.tree.root_half_width = 0.1
         hip.UniformFmm(x, x, opts)
+
+
+def test_operator_tables_blob_round_trip(hip, tmp_path):
+    # Versioned blob of the translation tables: loads back bit for bit for
+    # the same (order, depth, frame) keys and refuses anything else.
+    path = str(tmp_path / "tables.bin")
+    built = hip.build_operator_tables(4, 3, 1.0)
+    hip.save_operator_tables(path, 4, 3, 1.0)
+    loaded = hip.load_operator_tables(path, 4, 3, 1.0)
+    for key in ("class_offsets", "m2l", "ms", "ls", "m2m", "l2l"):
+        np.testing.assert_array_equal(loaded[key], built[key], err_msg=key)
+    assert len(loaded["class_offsets"]) == 3 * 316
+    with pytest.raises(ValueError, match="another order, depth or frame"):
+        hip.load_operator_tables(path, 4, 4, 1.0)
+    with pytest.raises(ValueError, match="another order, depth or frame"):
+        hip.load_operator_tables(path, 4, 3, 2.0)
+    blob = bytearray(open(path, "rb").read())
+    blob[len(blob) // 2] ^= 0x10
+    bad = str(tmp_path / "flipped.bin")
+    open(bad, "wb").write(bytes(blob))
+    with pytest.raises(ValueError, match="checksum"):
+        hip.load_operator_tables(bad, 4, 3, 1.0)
+    open(bad, "wb").write(bytes(blob[:100]))
+    with pytest.raises(ValueError):
+        hip.load_operator_tables(bad, 4, 3, 1.0)
+    with pytest.raises(ValueError, match="no such file"):
+        hip.load_operator_tables(str(tmp_path / "missing.bin"), 4, 3, 1.0)
+
+
+def test_operator_tables_content_against_reference(hip):
+    # M2L classes against the compiled reference's matrices (golden), the
+    # scalings against their closed form w_l^-|alpha|, w_l^-(|beta|+1).
+    g = golden("ref_m2l.npz")
+    for p in (4, 8):
+        t = hip.build_operator_tables(p, 2, 1.0)
+        n = (p + 1) * (p + 2) * (p + 3) // 6
+        offs = t["class_offsets"].reshape(-1, 3)
+        mats = t["m2l"].reshape(-1, n, n)  # [class][alpha][beta] (column-major T[beta + n*alpha])
+        for key in g.files:
+            if not key.startswith(f"p{p}/matrix_"):
+                continue
+            off = tuple(int(v) for v in key.split("matrix_")[1].split("_"))
+            k = [tuple(o) for o in offs].index(off)
+            ref = g[key]
+            np.testing.assert_allclose(mats[k].T, ref, rtol=2e-13, atol=1e-15 * np.abs(ref).max())
+        deg = np.concatenate([[d] * ((d + 1) * (d + 2) // 2) for d in range(p + 1)])
+        for level in range(3):
+            w = 1.0 / 2 ** level
+            np.testing.assert_allclose(t["ms"].reshape(3, n)[level], w ** -deg.astype(float), rtol=1e-15)
+            np.testing.assert_allclose(t["ls"].reshape(3, n)[level], w ** -(deg + 1.0), rtol=1e-15)
+    # FP64 frame: physical root width W scales the same tables.
+    a = hip.build_operator_tables(3, 2, 1.0)
+    b = hip.build_operator_tables(3, 2, 4.0)
+    np.testing.assert_array_equal(a["m2l"], b["m2l"])
+    n = 20
+    np.testing.assert_allclose(b["ms"].reshape(3, n), a["ms"].reshape(3, n) * 4.0 ** -np.concatenate(
+        [[d] * ((d + 1) * (d + 2) // 2) for d in range(4)]), rtol=1e-15)
