@@ -121,6 +121,8 @@ struct DevicePlan {
   int p2p_tile_count = 0;
   int2* p2p_tiles4 = nullptr; // the same with the potential: row = 4*target + (component or 3 = potential)
   int p2p_tile4_count = 0;
+  int2* p2p_tiles16 = nullptr; // (leaf, first target) of each 16-target tile (targets-as-rows kernel)
+  int p2p_tile16_count = 0;
   // basis tables
   int4* p2m_tab = nullptr;  // per alpha: index(alpha-e_c) or -1 for c=0..2, w = sign (+1/-1)
   // operators
@@ -175,7 +177,8 @@ struct DevicePlan {
   int apad = 0, bpad = 0;
   // Launch choices (set by the autotuner; 0 = built-in default).
   int m2l_shape = 0;    // MFMA M2L tile shape code (kM2LShapes)
-  int p2p_variant = 0;  // near field at K % 32 == 0 (FP32): 1 vector ALU, 2 matrix cores
+  int p2p_variant = 0;  // near field at K % 32 == 0 (FP32): 1 vector ALU, 2 matrix cores with
+                        // (target, component) rows, 3 (= default) matrix cores with target rows
 };
 
 // MFMA M2L tile shapes of the harmonic form (<= 3 row tiles): the code is

@@ -328,12 +328,145 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePl
   }
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// List-1 P2P on the matrix cores with targets as rows
+// (v_mfma_f32_16x16x4_f32), for FP32 plans with K a multiple of 32.  One
+// wave owns 16 targets of one leaf and 32 states; the depth of a step is 4
+// sources.  Lane l forms the pair (target l%16, source l/16) once -- r, |r|,
+// the 6 entries of the symmetric D = f (3 u u^T - I) and, for the potential,
+// f r -- and the products run per output component c as separate 16 x 16
+// accumulator tiles over 2 column halves:
+//   H_c[t][k] += sum_{c'} D_{c c'}(t, s) m[s][c'][k],
+// 9 products per 4 sources and 16 states (12 with the potential row).  Lane
+// layout (CDNA3/4 16x16x4 f32): A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D[i][j] in lane j + 16 floor(i/4), register i % 4.  Identity pair
+// and padding contribute zero; coincident pairs give NaN (r = 0).  Blocked
+// summation per 32 sources as in the 32 x 32 kernel.
+template <int kComps>
+__global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
+                                                                     int tile_count, int K) {
+  const int wave = int(threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int item = int(blockIdx.x) * kP2PWaves + wave;
+  if (item >= tile_count) return;
+  const int2 tile = tiles[item];
+  const int leaf = tile.x;
+  const int2 trange = d.leaf_tgt[leaf];
+  const int col = lane & 15, q = lane >> 4;  // A row / B and D column; depth slot
+  const bool row_ok = trange.x + tile.y + col < trange.y;
+  const int t = row_ok ? trange.x + tile.y + col : trange.x;
+  const PairPos<float> tp = PairPos<float>::load(d.tgt_pos, d.nt, t);
+  const int self = row_ok ? d.self[t] : -2;
+  const int kc0 = int(blockIdx.y) * 32;
+  const float c4pi = 0.079577471545947667884441881686257181f;
+  const float* sm = static_cast<const float*>(d.sm);
+  f32x4 acc[kComps][2];
+#pragma unroll
+  for (int c = 0; c < kComps; ++c)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) acc[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int nb = 1 << d.L;
+  int lx, ly, lz;
+  demorton3(uint32_t(leaf), d.L, lx, ly, lz);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = lx + dx, y = ly + dy, z = lz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+        const int2 sr = d.leaf_src[morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L)];
+        for (int jb = sr.x; jb < sr.y; jb += 32) {
+          const int je = min(sr.y, jb + 32);
+          f32x4 part[kComps][2];
+#pragma unroll
+          for (int c = 0; c < kComps; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+          for (int j0 = jb; j0 < je; j0 += 4) {
+            const int s = j0 + q;
+            const bool src_ok = s < je;
+            const int sl = src_ok ? s : sr.x;
+            float rx, ry, rz;
+            tp.diff(PairPos<float>::load(d.src_pos, d.ns, sl), rx, ry, rz);
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float rinv = inv_sqrt(r2);
+            const float f = rinv * rinv * rinv * c4pi;
+            const float ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
+            const bool use = row_ok & src_ok & (s != self);
+            const float f3 = 3.0f * f;
+            const float d00 = use ? f3 * ux * ux - f : 0.0f;
+            const float d01 = use ? f3 * ux * uy : 0.0f;
+            const float d02 = use ? f3 * ux * uz : 0.0f;
+            const float d11 = use ? f3 * uy * uy - f : 0.0f;
+            const float d12 = use ? f3 * uy * uz : 0.0f;
+            const float d22 = use ? f3 * uz * uz - f : 0.0f;
+            const float* ms = sm + int64_t(sl) * 3 * K + kc0 + col;
+            float b[3][2];
+#pragma unroll
+            for (int cp = 0; cp < 3; ++cp)
+#pragma unroll
+              for (int h = 0; h < 2; ++h) b[cp][h] = src_ok ? ms[cp * K + 16 * h] : 0.0f;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d00, b[0][h], part[0][h], 0, 0, 0);
+              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d01, b[1][h], part[0][h], 0, 0, 0);
+              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d02, b[2][h], part[0][h], 0, 0, 0);
+              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d01, b[0][h], part[1][h], 0, 0, 0);
+              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d11, b[1][h], part[1][h], 0, 0, 0);
+              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d12, b[2][h], part[1][h], 0, 0, 0);
+              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d02, b[0][h], part[2][h], 0, 0, 0);
+              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d12, b[1][h], part[2][h], 0, 0, 0);
+              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d22, b[2][h], part[2][h], 0, 0, 0);
+            }
+            if constexpr (kComps == 4) {
+              const float p0 = use ? f * rx : 0.0f, p1 = use ? f * ry : 0.0f, p2 = use ? f * rz : 0.0f;
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, b[0][h], part[3][h], 0, 0, 0);
+                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(p1, b[1][h], part[3][h], 0, 0, 0);
+                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(p2, b[2][h], part[3][h], 0, 0, 0);
+              }
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < kComps; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) acc[c][h] += part[c][h];
+        }
+      }
+  // acc[c][h][i]: target tile.y + 4q + i, state kc0 + 16h + col.
+  float* Hn = static_cast<float*>(d.Hn);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t ot = int64_t(trange.x) + tile.y + 4 * q + i;
+    if (ot >= trange.y) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = kc0 + 16 * h + col;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Hn[(ot * 3 + c) * K + k] = acc[c][h][i];
+      if constexpr (kComps == 4) static_cast<float*>(d.Pn)[ot * K + k] = acc[3][h][i];
+    }
+  }
+}
+
 template <class S>
 void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
   if (potential && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
   if constexpr (sizeof(S) == 4) {
-    if (K % 32 == 0 && d.p2p_variant != 1) {
+    if (K % 32 == 0 && d.p2p_variant != 1 && d.p2p_variant != 2) {
+      const int count = d.p2p_tile16_count;
+      if (count == 0) return;
+      const dim3 grid(unsigned((count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
+      if (potential)
+        p2p_mfma16_kernel<4><<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, count, K);
+      else
+        p2p_mfma16_kernel<3><<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, count, K);
+      HIPFMM_CHECK(hipGetLastError());
+      return;
+    }
+    if (K % 32 == 0 && d.p2p_variant == 2) {
       const int count = potential ? d.p2p_tile4_count : d.p2p_tile_count;
       if (count == 0) return;
       const dim3 grid(unsigned((count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));

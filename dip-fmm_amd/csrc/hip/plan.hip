@@ -65,6 +65,8 @@ struct hipfmm_plan {
   // leaves and the inverse identity map (sorted source -> sorted target).
   int2* adj_tiles = nullptr;
   int adj_tile_count = 0;
+  int2* adj_tiles16 = nullptr;
+  int adj_tile16_count = 0;
   int* adj_self = nullptr;
   bool adj_self_valid = true;  // false when two targets share one source
   bool potential_used = false;  // keep the near-field potential buffers in the state
@@ -320,6 +322,15 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   d.p2p_tile4_count = int(tiles4.size());
   d.p2p_tiles4 = upload(P, tiles4.data(), std::max<size_t>(tiles4.size(), 1));
   d.potential_scale = desc.potential_scale > 0.0 ? desc.potential_scale : 1.0;
+  auto tiles16_of = [&](const std::vector<int2>& ranges16, int& count) {
+    std::vector<int2> t16;
+    for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
+      for (int t0 = 0; t0 < ranges16[leaf].y - ranges16[leaf].x; t0 += 16) t16.push_back(int2{int(leaf), t0});
+    count = int(t16.size());
+    return upload(P, t16.data(), std::max<size_t>(t16.size(), 1));
+  };
+  d.p2p_tiles16 = tiles16_of(lt, d.p2p_tile16_count);
+  P->adj_tiles16 = tiles16_of(ls, P->adj_tile16_count);
   std::vector<int2> adj_tiles;
   for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
     for (int r0 = 0; r0 < 3 * (ls[leaf].y - ls[leaf].x); r0 += 32) adj_tiles.push_back(int2{int(leaf), r0});
@@ -594,6 +605,8 @@ DevicePlan role_exchanged(const hipfmm_plan* P) {
   a.self = P->adj_self;
   a.p2p_tiles = P->adj_tiles;
   a.p2p_tile_count = P->adj_tile_count;
+  a.p2p_tiles16 = P->adj_tiles16;
+  a.p2p_tile16_count = P->adj_tile16_count;
   return a;
 }
 
@@ -944,10 +957,24 @@ int hipfmm_plan_autotune(hipfmm_plan* plan, int batch) {
       best_of([&](int c) { P->d.m2l_shape = c; }, shapes, [&] { return P->timings.m2l_kernel_seconds; });
     }
     if (P->d.precision == 32 && batch % 32 == 0)
-      best_of([&](int c) { P->d.p2p_variant = c; }, {2, 1}, [&] { return P->timings.p2p_seconds; });
+      best_of([&](int c) { P->d.p2p_variant = c; }, {3, 2, 1}, [&] { return P->timings.p2p_seconds; });
     P->stats.m2l_tile_shape = uint64_t(P->d.m2l_shape);
     P->stats.p2p_kernel_variant = uint64_t(P->d.p2p_variant);
     P->stats.autotune_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int hipfmm_plan_set_launch_choices(hipfmm_plan* plan, int m2l_tile_shape, int p2p_kernel_variant) {
+  return guarded([&] {
+    require(plan != nullptr, "plan is null");
+    const bool shape_ok = m2l_tile_shape == 0 || std::find(std::begin(hipfmm::kM2LShapes), std::end(hipfmm::kM2LShapes),
+                                                           m2l_tile_shape) != std::end(hipfmm::kM2LShapes);
+    require(shape_ok, "unknown M2L tile shape code");
+    require(p2p_kernel_variant >= 0 && p2p_kernel_variant <= 3, "unknown near-field kernel variant");
+    plan->d.m2l_shape = m2l_tile_shape;
+    plan->d.p2p_variant = p2p_kernel_variant;
+    plan->stats.m2l_tile_shape = uint64_t(m2l_tile_shape);
+    plan->stats.p2p_kernel_variant = uint64_t(p2p_kernel_variant);
   });
 }
 

@@ -660,6 +660,10 @@ HipPlanStatistics UniformFmm::hip_plan_statistics() const {
 
 void UniformFmm::autotune(int batch) { check(hipfmm_plan_autotune(plan_, batch)); }
 
+void UniformFmm::set_launch_choices(int m2l_tile_shape, int p2p_kernel_variant) {
+  check(hipfmm_plan_set_launch_choices(plan_, m2l_tile_shape, p2p_kernel_variant));
+}
+
 std::vector<double> UniformFmm::multipole(int node, int state) const {
   std::vector<double> out(static_cast<std::size_t>(basis_.size()));
   check(hipfmm_plan_coefficients(plan_, 0, node, state, out.data()));

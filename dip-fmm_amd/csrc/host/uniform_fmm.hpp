@@ -201,6 +201,7 @@ class UniformFmm {
   // Time the launch variants at `batch` states and keep the fastest
   // (hipfmm_plan_autotune).
   void autotune(int batch);
+  void set_launch_choices(int m2l_tile_shape, int p2p_kernel_variant);
   // The C-ABI plan, for callers that group several plans into one launch
   // (hipfmm_group_create).
   hipfmm_plan* native_plan() const { return plan_; }

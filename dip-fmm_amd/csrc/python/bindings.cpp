@@ -444,6 +444,8 @@ m.def("hip_full_available", &hip_full_available);
           py::arg("batch"),
           "Time the matrix-core M2L tile shapes and the near-field kernels at `batch` states and keep the "
           "fastest; the choices appear in hip_plan_statistics.")
+      .def("set_launch_choices", &UniformFmm::set_launch_choices, py::arg("m2l_tile_shape"),
+           py::arg("p2p_kernel_variant"), "Pin the launch choices (codes as in hip_plan_statistics).")
       .def_property_readonly("m2l_mode", &UniformFmm::m2l_mode)
       .def_property_readonly("tree", &UniformFmm::tree, py::return_value_policy::reference_internal)
       .def_property_readonly("expansion_order", [](const UniformFmm& f) { return f.basis().order(); })

@@ -138,7 +138,8 @@ typedef struct hipfmm_stats {
   /* Launch choices in use (hipfmm_plan_autotune; no counterpart in the
    * reference): the matrix-core M2L tile shape code, 10 * column tiles per
    * wave + ring stages (0 = built-in default), and the near-field kernel at
-   * batch % 32 == 0 (0 default, 1 vector ALU, 2 matrix cores). */
+   * batch % 32 == 0 (0 default, 1 vector ALU, 2 matrix cores with (target,
+   * component) rows, 3 matrix cores with target rows -- the default). */
   uint64_t m2l_tile_shape;
   uint64_t p2p_kernel_variant;
   double autotune_seconds; /* host time of the last hipfmm_plan_autotune */
@@ -240,6 +241,12 @@ int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode);
  * class: every candidate meets the same accuracy bars.  Overwrites the
  * plan's coefficients of the last evaluate. */
 int hipfmm_plan_autotune(hipfmm_plan* plan, int batch);
+
+/* Set the launch choices directly (values as reported in hipfmm_stats: a
+ * tile shape code 43, 24, 34 or 26, or 0 for the default; a near-field
+ * variant 0..3), e.g. to reuse an earlier autotune result.  No counterpart in
+ * the reference. */
+int hipfmm_plan_set_launch_choices(hipfmm_plan* plan, int m2l_tile_shape, int p2p_kernel_variant);
 
 /* Multi-geometry launch; no counterpart in the reference.  A group records
  * the device-memory evaluates of `count` distinct plans (fields only, each

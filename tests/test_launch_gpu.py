@@ -58,7 +58,7 @@ def test_autotune_choices_meet_the_bar(hip, mode):
     st = f.hip_plan_statistics
     print(f"  shape {st['m2l_tile_shape']} near-field variant {st['p2p_kernel_variant']} "
           f"in {st['autotune_seconds']:.2f}s", flush=True)
-    assert st["m2l_tile_shape"] in SHAPES and st["p2p_kernel_variant"] in (1, 2)
+    assert st["m2l_tile_shape"] in SHAPES and st["p2p_kernel_variant"] in (1, 2, 3)
     after = f.evaluate_batch(ms)
     ref = O.CpuFmm(x, x, 6, 3, 64, fixed_ids=np.arange(n))
     for k in (0, K - 1):
@@ -152,3 +152,38 @@ def test_group_rejects_duplicates_and_bad_batches(hip):
         hip.HipGroup([f, f], [m.data_ptr()] * 2, [h.data_ptr()] * 2, [4, 4])
     with pytest.raises(ValueError, match="batch"):
         hip.HipGroup([f], [m.data_ptr()], [h.data_ptr()], [65])
+
+
+@pytest.mark.parametrize("output", ["field", "both"])
+def test_near_field_variants_agree(hip, output):
+    # The three near-field kernels at K = 32 (vector ALU; matrix cores with
+    # (target, component) rows; matrix cores with target rows) against the
+    # FP64 FMM and each other, fields and potentials.
+    rng = np.random.default_rng(980)
+    n, K = 9000, 32
+    x = rng.uniform(0, 1, size=(n, 3))
+    ms = rng.normal(size=(K, n, 3))
+    announce("near-field variants", output)
+    f = make(hip, x, x, 4, 3, 32, ids=np.arange(n))
+    ref = O.CpuFmm(x, x, 4, 3, 64, fixed_ids=np.arange(n))
+    want = [ref.evaluate(ms[k], np.arange(n), output="both") for k in (0, K - 1)]
+    got = {}
+    for variant in (1, 2, 3):
+        f.set_launch_choices(0, variant)
+        assert f.hip_plan_statistics["p2p_kernel_variant"] == variant
+        if output == "field":
+            got[variant] = {"H": f.evaluate_batch(ms), "phi": None}
+        else:
+            got[variant] = f.evaluate_batch_outputs(ms, output="both")
+        for i, k in enumerate((0, K - 1)):
+            e = rel_l2(got[variant]["H"][k], want[i]["H"])
+            print(f"  variant {variant} state {k}: H vs FP64 FMM {e:.3e}", flush=True)
+            assert e <= 1e-5
+            if output == "both":
+                assert rel_l2(got[variant]["phi"][k], want[i]["phi"]) <= 1e-5
+    for variant in (2, 3):
+        assert rel_l2(got[variant]["H"], got[1]["H"]) <= 2e-6
+    with pytest.raises(ValueError, match="near-field kernel variant"):
+        f.set_launch_choices(0, 7)
+    with pytest.raises(ValueError, match="tile shape"):
+        f.set_launch_choices(55, 0)
