@@ -170,17 +170,22 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    m2l_kernel = []
+    # Every M2L launch is timed by its own pair of HIP events on the plan's
+    # stream; the difference of the running totals around the timed region
+    # gives that region's average launch time without synchronising steps.
+    before = fmm.device_timings()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         step()
-        t = fmm.device_timings()  # waits for this evaluate; records its stage events
-        m2l_kernel.append(t["m2l_kernel"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     ms_per_step = 1e3 * elapsed / args.steps
     value = N * K * args.steps / elapsed
     stage = fmm.device_timings()
+    launches = stage["m2l_kernel_launches"] - before["m2l_kernel_launches"]
+    if launches != args.steps:
+        raise RuntimeError(f"expected {args.steps} M2L launches in the timed region, saw {launches}")
+    m2l_kernel = [(stage["m2l_kernel_total"] - before["m2l_kernel_total"]) / launches]
     log(f"MFMA K={K}: {ms_per_step:.2f} ms/step, {value:.3e} evals/s; stages {stage}")
 
     pairs = stats["m2l_interaction_count"]

@@ -125,6 +125,7 @@ struct DevicePlan {
   int p2p_tile16_count = 0;
   // basis tables
   int4* p2m_tab = nullptr;  // per alpha: index(alpha-e_c) or -1 for c=0..2, w = sign (+1/-1)
+  int4* l2p_up = nullptr;   // per g of degree < p: index(g + e_c) for c = 0..2
   // operators
   void* m2l_T = nullptr;     // FP32/FP64 class matrices, column-major [class][alpha][beta]
   int* m2l_slot = nullptr;   // 343 slot -> class index or -1

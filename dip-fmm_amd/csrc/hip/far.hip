@@ -554,6 +554,120 @@ inline unsigned tile_threads(int row_tiles, int KP) {
   return unsigned(threads);
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// L2P on the matrix cores (FP32 plans, K a multiple of 32): for 16 targets
+// of one leaf and 32 states,
+//   H_c[t][k] = Hn[t][c][k] - sum_g mono_g(t) L[g + e_c][k]      (|g| < p)
+//   phi[t][k] = W (Pn[t][k] + sum_b mono_b(t) L[b][k])           (|b| <= p)
+// with mono_g(t) = d_t^g / g! (src/static_operators.cpp:300-335), as
+// v_mfma_f32_16x16x4_f32 products: A = monomials (rows = targets, depth = 4
+// monomials), B = rows of the leaf local, one accumulator tile per output
+// component and 16-state half; the monomial operand is shared by the three
+// components.  One wave per 16-target tile (the near-field tiles), four per
+// workgroup; per wave LDS holds the axis powers and the monomial table
+// mono[g][t].  Writes out[k][user target][3] (and pot[k][user target]).
+template <bool kPot>
+__global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float* __restrict__ out,
+                                                       float* __restrict__ pot, int K) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = d.n, p = d.p, NM = kPot ? d.n : d.nm, PW = p + 1;
+  const int wave = int(threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  float* pw = reinterpret_cast<float*>(smem_raw) + wave * (48 * PW);       // [t][axis][i]
+  float* mono = reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + wave * (NM * 16);  // [g][t]
+  const int item = int(blockIdx.x) * 4 + wave;
+  const bool valid = item < d.p2p_tile16_count;
+  const int2 tile = valid ? d.p2p_tiles16[item] : int2{0, 0};
+  const int2 trange = valid ? d.leaf_tgt[tile.x] : int2{0, 0};
+  const int t0 = trange.x + tile.y;
+  const int cnt = valid ? min(16, trange.y - t0) : 0;
+  const auto* rel = static_cast<const float4*>(d.tgt_rel);
+  if (lane < 48) {
+    const int t = lane / 3, a = lane % 3;
+    float x = 0.0f;
+    if (t < cnt) {
+      const float4 r = rel[t0 + t];
+      x = a == 0 ? r.x : (a == 1 ? r.y : r.z);
+    }
+    float v = 1.0f;
+    pw[lane * PW] = 1.0f;
+    for (int i = 1; i < PW; ++i) {
+      v = v * x / float(i);
+      pw[lane * PW + i] = v;
+    }
+  }
+  __syncthreads();
+  for (int w = lane; w < NM * 16; w += 64) {
+    const int g = w >> 4, t = w & 15;
+    const int4 e = d.p2m_tab[n + g];
+    const float* pt = pw + t * 3 * PW;
+    mono[w] = pt[e.x] * pt[PW + e.y] * pt[2 * PW + e.z];
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int col = lane & 15, q = lane >> 4;
+  const int kc0 = int(blockIdx.y) * 32;
+  const int64_t node = level_offset(d.L) + tile.x;
+  const float* Lb = static_cast<const float*>(d.Lc) + node * int64_t(n) * K + kc0 + col;
+  f32x4 acc[3][2], accp[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    acc[0][h] = acc[1][h] = acc[2][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    accp[h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
+  const int nm = d.nm;
+  for (int g0 = 0; g0 < nm; g0 += 4) {
+    const int g = g0 + q;
+    const bool ok = g < nm;
+    const float a = ok ? mono[g * 16 + col] : 0.0f;
+    const int4 up = ok ? d.l2p_up[g] : int4{0, 0, 0, 0};
+    const int rows[3] = {up.x, up.y, up.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float b = ok ? Lb[int64_t(rows[c]) * K + 16 * h] : 0.0f;
+        acc[c][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[c][h], 0, 0, 0);
+      }
+  }
+  if constexpr (kPot) {
+    for (int b0 = 0; b0 < n; b0 += 4) {
+      const int b = b0 + q;
+      const bool ok = b < n;
+      const float a = ok ? mono[b * 16 + col] : 0.0f;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float bv = ok ? Lb[int64_t(b) * K + 16 * h] : 0.0f;
+        accp[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, accp[h], 0, 0, 0);
+      }
+    }
+  }
+  // acc[c][h][r]: target t0 + 4q + r, state kc0 + 16h + col.
+  const float* Hn = static_cast<const float*>(d.Hn);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int tl = 4 * q + r;
+    if (tl >= cnt) continue;
+    const int ts = t0 + tl;
+    const int user = d.tgt_perm[ts];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = kc0 + 16 * h + col;
+      if (!kPot || out != nullptr) {
+        float* o = out + (int64_t(k) * d.nt + user) * 3;
+        const float* hn = Hn + int64_t(ts) * 3 * K + k;
+        o[0] = hn[0] - acc[0][h][r];
+        o[1] = hn[K] - acc[1][h][r];
+        o[2] = hn[2 * K] - acc[2][h][r];
+      }
+      if constexpr (kPot) {
+        const float pn = static_cast<const float*>(d.Pn)[int64_t(ts) * K + k];
+        pot[int64_t(k) * d.nt + user] = float(d.potential_scale) * (pn + accp[h][r]);
+      }
+    }
+  }
+}
+
 }  // namespace
 
 template <class S>
@@ -611,6 +725,19 @@ void launch_l2p_unsort(const DevicePlan& d, S* fields, S* potentials, int K, hip
   const bool pot = potentials != nullptr;
   if (!pot && fields == nullptr) throw invalid_argument("no output requested");
   if (pot && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
+  if constexpr (sizeof(S) == 4) {
+    const int NM = pot ? d.n : d.nm;
+    const size_t lds = (size_t(4) * 48 * (d.p + 1) + size_t(4) * NM * 16) * sizeof(float);
+    if (K % 32 == 0 && d.l2p_up != nullptr && d.p2p_tile16_count > 0 && lds <= 65536) {
+      const dim3 grid(unsigned((d.p2p_tile16_count + 3) / 4), unsigned(K / 32));
+      if (pot)
+        l2p_mfma_kernel<true><<<grid, 256, lds, s>>>(d, fields, potentials, K);
+      else
+        l2p_mfma_kernel<false><<<grid, 256, lds, s>>>(d, fields, nullptr, K);
+      HIPFMM_CHECK(hipGetLastError());
+      return;
+    }
+  }
   const int kc = state_chunk(K, [&](int c) { return l2p_layout<S>(d, c, pot).bytes; });
   const MonoLayout l = l2p_layout<S>(d, kc, pot);
   const unsigned threads = std::max(tile_threads(l.TCP / kL2PRT, l.KP), unsigned(round_up(3 * l.TCP, 64)));

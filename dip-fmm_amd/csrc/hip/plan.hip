@@ -78,6 +78,15 @@ struct hipfmm_plan {
   hipfmm_stats stats{};
   hipfmm_timings timings{};
   bool timings_pending = false;
+  // Ring of per-launch M2L kernel events: evaluates never wait on the host
+  // for earlier ones; a slot is read back when it is reused or when the
+  // timings are asked for.
+  static constexpr int kRing = 64;
+  hipEvent_t m2l_ring[kRing][2] = {};
+  bool m2l_ring_pending[kRing] = {};
+  uint64_t m2l_launch_count = 0;
+  double m2l_total_seconds = 0.0;
+  uint64_t m2l_total_launches = 0;
   bool last_host = false;
   int last_K = 0;
   size_t persistent_bytes = 0;
@@ -354,6 +363,23 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   d.self = upload(P, self.data(), self.size());
   const auto tab = basis_tables(p);
   d.p2m_tab = upload(P, tab.data(), tab.size());
+  // Raising table of the matrix-core L2P: g (degree < p) -> g + e_c.
+  {
+    std::vector<int4> up(std::max(d.nm, 1), int4{0, 0, 0, 0});
+    for (int g = 0; g < d.nm; ++g) {
+      const int4 e = tab[n + g];
+      int idx[3] = {-1, -1, -1};
+      for (int b = 0; b < n; ++b) {
+        const int4 f = tab[n + b];
+        if (f.x == e.x + 1 && f.y == e.y && f.z == e.z) idx[0] = b;
+        if (f.x == e.x && f.y == e.y + 1 && f.z == e.z) idx[1] = b;
+        if (f.x == e.x && f.y == e.y && f.z == e.z + 1) idx[2] = b;
+      }
+      if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0) throw std::logic_error("basis raising table");
+      up[g] = int4{idx[0], idx[1], idx[2], 0};
+    }
+    d.l2p_up = upload(P, up.data(), up.size());
+  }
   // M2L class matrices keyed by offset slot.
   std::vector<int> slot(hipfmm::kOffsetSlots, -1), slot_of_class(std::max(desc.m2l_class_count, 1), -1);
   for (int c = 0; c < desc.m2l_class_count; ++c) {
@@ -517,9 +543,14 @@ void ensure_capacity(hipfmm_plan* P, int K, bool potential) {
   DevicePlan& d = P->d;
   if (potential && !P->potential_used) {
     P->potential_used = true;
+    HIPFMM_CHECK(hipStreamSynchronize(P->s_far));
+    HIPFMM_CHECK(hipStreamSynchronize(P->s_near));
     free_state(P);  // reallocated below with the potential buffers
   }
   if (K <= d.kcap) return;
+  // Earlier evaluates may still be running on the plan's streams.
+  HIPFMM_CHECK(hipStreamSynchronize(P->s_far));
+  HIPFMM_CHECK(hipStreamSynchronize(P->s_near));
   free_state(P);
   const int cap = std::min(hipfmm::kMaxBatch, std::max(K, 1));
   const size_t cs = size_t(d.nodes) * d.n * cap;
@@ -571,6 +602,15 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return 0.0f;
   }
   return ms;
+}
+
+// Read back one ring slot of M2L kernel events (waits for it).
+void drain_m2l_slot(hipfmm_plan* P, int slot) {
+  if (!P->m2l_ring_pending[slot]) return;
+  HIPFMM_CHECK(hipEventSynchronize(P->m2l_ring[slot][1]));
+  P->m2l_total_seconds += 1e-3 * elapsed(P->m2l_ring[slot][0], P->m2l_ring[slot][1]);
+  P->m2l_total_launches += 1;
+  P->m2l_ring_pending[slot] = false;
 }
 
 void finalize_timings(hipfmm_plan* P) {
@@ -646,7 +686,8 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   }
   require(!capture || memspace == HIPFMM_DEVICE, "a captured evaluate takes device operands");
   if (!capture) {
-    if (P->timings_pending) finalize_timings(P);
+    // No wait on earlier evaluates here: their stage events are read back
+    // only when timings are asked for, so the host can enqueue ahead.
     (void)hipGetLastError();
     ensure_capacity<S>(P, K, want_pot);
   } else {
@@ -741,11 +782,21 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     hipfmm::launch_m2l_split(d, K, P->m2l_mode, s);
   }
   mark(kEvM2LKernel0, s);
+  const int ring_slot = int(P->m2l_launch_count % hipfmm_plan::kRing);
+  if (!capture) {
+    drain_m2l_slot(P, ring_slot);
+    HIPFMM_CHECK(hipEventRecord(P->m2l_ring[ring_slot][0], s));
+  }
   if (mfma)
     hipfmm::launch_m2l_mfma(dm, K, P->m2l_mode, s);
   else
     hipfmm::launch_m2l_valu<S>(dm, K, s);
   mark(kEvM2LKernel1, s);
+  if (!capture) {
+    HIPFMM_CHECK(hipEventRecord(P->m2l_ring[ring_slot][1], s));
+    P->m2l_ring_pending[ring_slot] = true;
+    ++P->m2l_launch_count;
+  }
   if (d.nc > 0 && d.L >= 2) hipfmm::launch_m2l_expand<S>(d, K, s);
   mark(kEvM2L, s);
   for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(d, level, K, s);
@@ -835,6 +886,8 @@ int hipfmm_plan_create(const hipfmm_plan_desc* desc, hipfmm_plan** plan) {
     HIPFMM_CHECK(hipStreamCreateWithFlags(&P->s_far, hipStreamNonBlocking));
     HIPFMM_CHECK(hipStreamCreateWithFlags(&P->s_near, hipStreamNonBlocking));
     for (auto& ev : P->ev) HIPFMM_CHECK(hipEventCreate(&ev));
+    for (auto& pair : P->m2l_ring)
+      for (auto& ev : pair) HIPFMM_CHECK(hipEventCreate(&ev));
     P->device_moment_scale = desc->device_moment_scale;
     P->m2l_mode = desc->m2l_mode;
     try {
@@ -992,7 +1045,10 @@ int hipfmm_plan_timings(const hipfmm_plan* plan, hipfmm_timings* out) {
     auto* P = const_cast<hipfmm_plan*>(plan);
     DeviceGuard guard(P->device);
     finalize_timings(P);
+    for (int slot = 0; slot < hipfmm_plan::kRing; ++slot) drain_m2l_slot(P, slot);
     *out = P->timings;
+    out->m2l_kernel_total_seconds = P->m2l_total_seconds;
+    out->m2l_kernel_launches = P->m2l_total_launches;
   });
 }
 
@@ -1033,6 +1089,9 @@ void hipfmm_plan_destroy(hipfmm_plan* plan) {
   if (plan->pinned_pot) (void)hipHostFree(plan->pinned_pot);
   for (auto& ev : plan->ev)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto& pair : plan->m2l_ring)
+    for (auto& ev : pair)
+      if (ev) (void)hipEventDestroy(ev);
   if (plan->s_far) (void)hipStreamDestroy(plan->s_far);
   if (plan->s_near) (void)hipStreamDestroy(plan->s_near);
   if (prev >= 0) (void)hipSetDevice(prev);

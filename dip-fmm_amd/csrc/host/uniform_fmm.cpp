@@ -415,6 +415,8 @@ void UniformFmm::record_timings() {
   l.total = t.total_seconds;
   l.m2l_kernel = t.m2l_kernel_seconds;
   l.evaluations = 1;
+  l.m2l_kernel_total = t.m2l_kernel_total_seconds;
+  l.m2l_kernel_launches = t.m2l_kernel_launches;
   auto& a = aggregate_timings_;
   a.h2d += t.h2d_seconds;
   a.p2m += t.p2m_seconds;

@@ -127,6 +127,10 @@ struct EvaluationTimings {
   double total{0.0};
   double m2l_kernel{0.0};  // M2L product kernel (one launch, all levels)
   std::uint64_t evaluations{0};
+  // Every M2L launch of the plan so far (own events per launch): summed
+  // seconds and launch count.
+  double m2l_kernel_total{0.0};
+  std::uint64_t m2l_kernel_launches{0};
 };
 
 // Device statistics (include/cdfmm/timings.hpp:182-222 subset).

@@ -78,6 +78,8 @@ py::dict timings_dict(const EvaluationTimings& t) {
   d["total"] = t.total;
   d["m2l_kernel"] = t.m2l_kernel;
   d["evaluations"] = t.evaluations;
+  d["m2l_kernel_total"] = t.m2l_kernel_total;
+  d["m2l_kernel_launches"] = t.m2l_kernel_launches;
   return d;
 }
 

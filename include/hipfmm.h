@@ -159,6 +159,11 @@ typedef struct hipfmm_timings {
   double d2h_seconds;
   double total_seconds;
   double m2l_kernel_seconds; /* the M2L product kernel alone (one launch, all levels) */
+  /* Every M2L kernel launch of the plan so far, each timed with its own HIP
+   * events on the plan's stream: summed durations and launch count (callers
+   * average over a region by differencing two reads). */
+  double m2l_kernel_total_seconds;
+  uint64_t m2l_kernel_launches;
 } hipfmm_timings;
 
 /* Message of the last failure on this thread ("" if none). */
