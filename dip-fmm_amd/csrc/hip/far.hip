@@ -556,6 +556,110 @@ inline unsigned tile_threads(int row_tiles, int KP) {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// P2M on the matrix cores (FP32 plans, K a multiple of 32).  With
+// g = alpha - e_c the multipoles of a leaf are
+//   M[alpha][k] = (-1)^|alpha| sum_{c: alpha_c > 0} C_c[alpha - e_c][k],
+//   C_c[g][k]   = sum_s mono_g(d_s) m[s][c][k]        (|g| < p)
+// (src/static_operators.cpp:186-220); the C_c are v_mfma_f32_16x16x4_f32
+// products with the monomials as rows (A, shared by the three components),
+// 4 sources of depth per step and 2 halves of 32 states.  One workgroup (4
+// waves) per leaf and 32 states; wave w owns monomial row tiles w, w+4, ...
+// (kTW of them).  Sources go through LDS 64 at a time as axis-power tables;
+// the C_c meet in LDS, where the sign-and-shift combination writes M.
+template <int kTW>
+__global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = d.n, nm = d.nm, PW = d.p;  // axis powers 0..p-1
+  const int NT = (nm + 15) / 16, NMP = NT * 16;
+  float* pw = reinterpret_cast<float*>(smem_raw);  // [64][3][PW]
+  float* Cl = pw + 64 * 3 * PW;                      // [3][NMP][32]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 15, q = lane >> 4;
+  const int leaf = blockIdx.x, kc0 = int(blockIdx.y) * 32;
+  const int2 range = d.leaf_src[leaf];
+  const int64_t node = level_offset(d.L) + leaf;
+  const auto* rel = static_cast<const float4*>(d.src_rel);
+  const float* sm = static_cast<const float*>(d.sm);
+  int4 ex[kTW];
+  bool row_ok[kTW];
+#pragma unroll
+  for (int j = 0; j < kTW; ++j) {
+    const int g = (wave + 4 * j) * 16 + col;
+    row_ok[j] = g < nm;
+    ex[j] = row_ok[j] ? d.p2m_tab[n + g] : int4{0, 0, 0, 0};
+  }
+  f32x4 acc[kTW][3][2];
+#pragma unroll
+  for (int j = 0; j < kTW; ++j)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[j][c][0] = acc[j][c][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int j0 = range.x; j0 < range.y; j0 += 64) {
+    const int cnt = min(64, range.y - j0);
+    __syncthreads();  // the previous chunk's tables are no longer read
+    if (tid < 192) {
+      const int sj = tid / 3, a = tid % 3;
+      float x = 0.0f;
+      if (sj < cnt) {
+        const float4 r = rel[j0 + sj];
+        x = a == 0 ? r.x : (a == 1 ? r.y : r.z);
+      }
+      float v = 1.0f;
+      float* row = pw + tid * PW;
+      row[0] = 1.0f;
+      for (int i = 1; i < PW; ++i) {
+        v = v * x / float(i);
+        row[i] = v;
+      }
+    }
+    __syncthreads();
+    for (int s0 = 0; s0 < cnt; s0 += 4) {
+      const int sj = s0 + q;
+      const bool ok = sj < cnt;
+      const float* ps = pw + (ok ? sj : 0) * 3 * PW;
+      const float* ms = sm + int64_t(j0 + (ok ? sj : 0)) * 3 * K + kc0 + col;
+      float b[3][2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) b[c][h] = ok ? ms[c * K + 16 * h] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < kTW; ++j) {
+        if (wave + 4 * j >= NT) break;
+        const float a = (ok && row_ok[j]) ? ps[ex[j].x] * ps[PW + ex[j].y] * ps[2 * PW + ex[j].z] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) acc[j][c][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[c][h], acc[j][c][h], 0, 0, 0);
+      }
+    }
+  }
+  // C_c[g][k] to LDS: acc[j][c][h][r] is row (wave + 4j)*16 + 4q + r, state 16h + col.
+#pragma unroll
+  for (int j = 0; j < kTW; ++j) {
+    if (wave + 4 * j >= NT) break;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Cl[(c * NMP + (wave + 4 * j) * 16 + 4 * q + r) * 32 + 16 * h + col] = acc[j][c][h][r];
+  }
+  __syncthreads();
+  float* M = static_cast<float*>(d.M) + node * int64_t(n) * K + kc0;
+  for (int w = tid; w < n * 32; w += 256) {
+    const int b = w >> 5, k = w & 31;
+    const int4 t = d.p2m_tab[b];
+    float v = 0.0f;
+    if (t.x >= 0) v += Cl[(0 * NMP + t.x) * 32 + k];
+    if (t.y >= 0) v += Cl[(1 * NMP + t.y) * 32 + k];
+    if (t.z >= 0) v += Cl[(2 * NMP + t.z) * 32 + k];
+    M[int64_t(b) * K + k] = t.w > 0 ? v : -v;
+  }
+}
+
 // L2P on the matrix cores (FP32 plans, K a multiple of 32): for 16 targets
 // of one leaf and 32 states,
 //   H_c[t][k] = Hn[t][c][k] - sum_g mono_g(t) L[g + e_c][k]      (|g| < p)
@@ -691,6 +795,21 @@ int state_chunk(int K, F bytes_for) {
 template <class S>
 void launch_p2m(const DevicePlan& d, int K, hipStream_t s) {
   if (d.leaves == 0) return;
+  if constexpr (sizeof(S) == 4) {
+    const int NT = (d.nm + 15) / 16, TW = (NT + 3) / 4;
+    const size_t lds = (size_t(64) * 3 * std::max(d.p, 1) + size_t(3) * NT * 16 * 32) * sizeof(float);
+    if (K % 32 == 0 && d.p >= 1 && TW <= 4 && lds <= 65536) {
+      const dim3 grid(unsigned(d.leaves), unsigned(K / 32));
+      switch (TW) {
+        case 1: p2m_mfma_kernel<1><<<grid, 256, lds, s>>>(d, K); break;
+        case 2: p2m_mfma_kernel<2><<<grid, 256, lds, s>>>(d, K); break;
+        case 3: p2m_mfma_kernel<3><<<grid, 256, lds, s>>>(d, K); break;
+        default: p2m_mfma_kernel<4><<<grid, 256, lds, s>>>(d, K); break;
+      }
+      HIPFMM_CHECK(hipGetLastError());
+      return;
+    }
+  }
   const int kc = state_chunk(K, [&](int c) { return p2m_layout<S>(d, c).bytes; });
   const MonoLayout l = p2m_layout<S>(d, kc);
   const unsigned threads = std::max(tile_threads(l.NP / kRA, l.KP), unsigned(round_up(3 * l.JC, 64)));

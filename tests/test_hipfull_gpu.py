@@ -404,3 +404,27 @@ def test_operator_cache_plans_match_built_plans(hip, prec, tmp_path):
     assert second.hip_plan_statistics["operator_tables_loaded"]
     np.testing.assert_array_equal(first.evaluate(m)["H"], plain)
     np.testing.assert_array_equal(second.evaluate(m)["H"], plain)
+
+
+def test_batched_expansions_match_oracle_fp32(hip):
+    # K = 32 on an FP32 plan: the matrix-core P2M's multipoles and the locals
+    # they lead to, per state, against the FP64 oracle (node coefficients in
+    # the operator frame, compared relative to each node's largest).
+    rng = np.random.default_rng(995)
+    n, K = 6000, 32
+    x = rng.uniform(0, 1, size=(n, 3))
+    ms = rng.normal(size=(K, n, 3))
+    announce("batched expansions fp32")
+    f = make(hip, x, x, 6, 3, 32, ids=np.arange(n))
+    f.evaluate_batch(ms)
+    for k in (0, 31):
+        o = O.CpuFmm(x, x, 6, 3, 32, fixed_ids=np.arange(n))
+        o.evaluate(ms[k])
+        for node in list(range(73, 585, 37)):
+            ref = o.coefficients("multipole", node)
+            got = f.multipole(node, k)
+            scale = np.abs(ref).max()
+            if scale == 0:
+                assert not np.any(got)
+                continue
+            assert np.abs(got - ref).max() <= 2e-5 * scale, (k, node)
