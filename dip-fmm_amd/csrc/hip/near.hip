@@ -342,7 +342,55 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // layout (CDNA3/4 16x16x4 f32): A[i][k] in lane i + 16k, B[k][j] in lane
 // j + 16k, D[i][j] in lane j + 16 floor(i/4), register i % 4.  Identity pair
 // and padding contribute zero; coincident pairs give NaN (r = 0).  Blocked
-// summation per 32 sources as in the 32 x 32 kernel.
+// summation per 32 sources as in the 32 x 32 kernel.  |r|^-1 is the hardware
+// reciprocal square root (1 ulp).
+// One step's operands of the targets-as-rows kernel for this lane's pair
+// (row target, source s): the 6 distinct entries of D (d00 d01 d02 d11 d12
+// d22), the potential row f r, and the moments m[s][c'][k] of both 16-state
+// halves.  Sources at or past `end` (and the identity pair) give zeros; the
+// load index is clamped to `first` so every access stays in the leaf.
+struct P2POperands {
+  float d[6];
+  float p[3];
+  float b[3][2];
+};
+
+template <int kComps>
+__device__ __forceinline__ P2POperands p2p_operands(const DevicePlan& d, const PairPos<float>& tp,
+                                                    const float* __restrict__ sm, bool row_ok, int self, int s,
+                                                    int end, int first, int K, int kcol) {
+  P2POperands o;
+  const bool src_ok = s < end;
+  const int sl = src_ok ? s : first;
+  float rx, ry, rz;
+  tp.diff(PairPos<float>::load(d.src_pos, d.ns, sl), rx, ry, rz);
+  const float* ms = sm + int64_t(sl) * 3 * K + kcol;
+#pragma unroll
+  for (int cp = 0; cp < 3; ++cp)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) o.b[cp][h] = src_ok ? ms[cp * K + 16 * h] : 0.0f;
+  const float r2 = rx * rx + ry * ry + rz * rz;
+  const float rinv = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32 (1 ulp); r = 0 gives inf, hence NaN below
+  const float f = rinv * rinv * rinv * 0.079577471545947667884441881686257181f;
+  const float ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
+  const bool use = row_ok & src_ok & (s != self);
+  const float f3 = 3.0f * f;
+  o.d[0] = use ? f3 * ux * ux - f : 0.0f;
+  o.d[1] = use ? f3 * ux * uy : 0.0f;
+  o.d[2] = use ? f3 * ux * uz : 0.0f;
+  o.d[3] = use ? f3 * uy * uy - f : 0.0f;
+  o.d[4] = use ? f3 * uy * uz : 0.0f;
+  o.d[5] = use ? f3 * uz * uz - f : 0.0f;
+  if constexpr (kComps == 4) {
+    o.p[0] = use ? f * rx : 0.0f;
+    o.p[1] = use ? f * ry : 0.0f;
+    o.p[2] = use ? f * rz : 0.0f;
+  } else {
+    o.p[0] = o.p[1] = o.p[2] = 0.0f;
+  }
+  return o;
+}
+
 template <int kComps>
 __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
                                                                      int tile_count, int K) {
@@ -358,7 +406,6 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const Device
   const PairPos<float> tp = PairPos<float>::load(d.tgt_pos, d.nt, t);
   const int self = row_ok ? d.self[t] : -2;
   const int kc0 = int(blockIdx.y) * 32;
-  const float c4pi = 0.079577471545947667884441881686257181f;
   const float* sm = static_cast<const float*>(d.sm);
   f32x4 acc[kComps][2];
 #pragma unroll
@@ -381,52 +428,30 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const Device
           for (int c = 0; c < kComps; ++c)
 #pragma unroll
             for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 2
+          // Software pipelined: the loads and pair geometry of step j0 + 4
+          // are issued before the products of step j0, so they overlap the
+          // matrix pipe.
+          P2POperands cur = p2p_operands<kComps>(d, tp, sm, row_ok, self, jb + q, je, sr.x, K, kc0 + col);
           for (int j0 = jb; j0 < je; j0 += 4) {
-            const int s = j0 + q;
-            const bool src_ok = s < je;
-            const int sl = src_ok ? s : sr.x;
-            float rx, ry, rz;
-            tp.diff(PairPos<float>::load(d.src_pos, d.ns, sl), rx, ry, rz);
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const float rinv = inv_sqrt(r2);
-            const float f = rinv * rinv * rinv * c4pi;
-            const float ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
-            const bool use = row_ok & src_ok & (s != self);
-            const float f3 = 3.0f * f;
-            const float d00 = use ? f3 * ux * ux - f : 0.0f;
-            const float d01 = use ? f3 * ux * uy : 0.0f;
-            const float d02 = use ? f3 * ux * uz : 0.0f;
-            const float d11 = use ? f3 * uy * uy - f : 0.0f;
-            const float d12 = use ? f3 * uy * uz : 0.0f;
-            const float d22 = use ? f3 * uz * uz - f : 0.0f;
-            const float* ms = sm + int64_t(sl) * 3 * K + kc0 + col;
-            float b[3][2];
-#pragma unroll
-            for (int cp = 0; cp < 3; ++cp)
-#pragma unroll
-              for (int h = 0; h < 2; ++h) b[cp][h] = src_ok ? ms[cp * K + 16 * h] : 0.0f;
+            const P2POperands nxt = p2p_operands<kComps>(d, tp, sm, row_ok, self, j0 + 4 + q, je, sr.x, K, kc0 + col);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d00, b[0][h], part[0][h], 0, 0, 0);
-              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d01, b[1][h], part[0][h], 0, 0, 0);
-              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d02, b[2][h], part[0][h], 0, 0, 0);
-              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d01, b[0][h], part[1][h], 0, 0, 0);
-              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d11, b[1][h], part[1][h], 0, 0, 0);
-              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d12, b[2][h], part[1][h], 0, 0, 0);
-              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d02, b[0][h], part[2][h], 0, 0, 0);
-              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d12, b[1][h], part[2][h], 0, 0, 0);
-              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(d22, b[2][h], part[2][h], 0, 0, 0);
-            }
-            if constexpr (kComps == 4) {
-              const float p0 = use ? f * rx : 0.0f, p1 = use ? f * ry : 0.0f, p2 = use ? f * rz : 0.0f;
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, b[0][h], part[3][h], 0, 0, 0);
-                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(p1, b[1][h], part[3][h], 0, 0, 0);
-                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(p2, b[2][h], part[3][h], 0, 0, 0);
+              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[0], cur.b[0][h], part[0][h], 0, 0, 0);
+              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], cur.b[0][h], part[1][h], 0, 0, 0);
+              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], cur.b[0][h], part[2][h], 0, 0, 0);
+              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], cur.b[1][h], part[0][h], 0, 0, 0);
+              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[3], cur.b[1][h], part[1][h], 0, 0, 0);
+              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], cur.b[1][h], part[2][h], 0, 0, 0);
+              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], cur.b[2][h], part[0][h], 0, 0, 0);
+              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], cur.b[2][h], part[1][h], 0, 0, 0);
+              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[5], cur.b[2][h], part[2][h], 0, 0, 0);
+              if constexpr (kComps == 4) {
+                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[0], cur.b[0][h], part[3][h], 0, 0, 0);
+                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[1], cur.b[1][h], part[3][h], 0, 0, 0);
+                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[2], cur.b[2][h], part[3][h], 0, 0, 0);
               }
             }
+            cur = nxt;
           }
 #pragma unroll
           for (int c = 0; c < kComps; ++c)
