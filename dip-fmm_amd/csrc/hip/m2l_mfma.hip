@@ -317,9 +317,14 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
   static_assert((kStages - 2) * kDmaPerStep <= 63, "vmcnt holds at most 63 outstanding operations");
   // Ring slots: the class-matrix slice, then the waves' source fragments.
   __shared__ uint4 ring[kStages * kSlot];
-  __shared__ int4 offs_all[kParityOffsets];
   __shared__ int4 offs[kParityOffsets];  // the offsets some column of this workgroup uses
-  __shared__ unsigned char used[W][kParityOffsets];
+  // The parity's offsets and the per-wave usage flags are read only by the
+  // compaction below, which ends with a barrier before the first DMA writes
+  // the ring; they live in the ring's first bytes.
+  static_assert(kParityOffsets * (16 + W) <= kStages * kSlot * 16, "compaction scratch fits the ring");
+  int4* const offs_all = reinterpret_cast<int4*>(ring);
+  unsigned char (*const used)[kParityOffsets] =
+      reinterpret_cast<unsigned char (*)[kParityOffsets]>(reinterpret_cast<unsigned char*>(ring) + kParityOffsets * 16);
   __shared__ int active_count;
   __shared__ StepTables<W, NB> tab;
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
@@ -554,11 +559,11 @@ void launch_mfma_cfg(const DevicePlan& d, int K, bool f16, hipStream_t s) {
 }
 
 // Tile shape per row-tile count MT: NB 32-column tiles per wave (NB * MT *
-// 16 accumulator registers, at most 192 so the fragments fit beside them) and
-// the ring depth that fits LDS (160 KiB per workgroup).  The shape only
-// regroups whole output columns, so every shape gives bitwise the same
-// locals.  HIPFMM_M2L_MFMA_SHAPE=NBxS (2x4, 3x4, 4x3, 2x6) overrides the choice among
-// the compiled shapes, for measurement.
+// 16 accumulator registers; at MT <= 3 the default 5 x 3 keeps its 240 in
+// AGPRs) and the ring depth that fits LDS (160 KiB per workgroup).  The
+// shape only regroups whole output columns, so every shape gives bitwise the
+// same locals.  HIPFMM_M2L_MFMA_SHAPE=NBxS (2x4, 3x4, 4x3, 5x3, 2x6)
+// overrides the choice among the compiled shapes, for measurement.
 inline int mfma_shape_override() {
   static const int v = [] {
     const char* e = std::getenv("HIPFMM_M2L_MFMA_SHAPE");
@@ -567,6 +572,7 @@ inline int mfma_shape_override() {
     if (std::string(e) == "3x4") return 34;
     if (std::string(e) == "4x3") return 43;
     if (std::string(e) == "2x6") return 26;
+    if (std::string(e) == "5x3") return 53;
     return 0;
   }();
   return v;
@@ -579,7 +585,8 @@ void launch_mfma_t(const DevicePlan& d, int K, bool f16, hipStream_t s) {
       case 24: launch_mfma_cfg<MT, 2, 4>(d, K, f16, s); return;
       case 34: launch_mfma_cfg<MT, 3, 4>(d, K, f16, s); return;
       case 26: launch_mfma_cfg<MT, 2, 6>(d, K, f16, s); return;
-      default: launch_mfma_cfg<MT, 4, 3>(d, K, f16, s); return;
+      case 43: launch_mfma_cfg<MT, 4, 3>(d, K, f16, s); return;
+      default: launch_mfma_cfg<MT, 5, 3>(d, K, f16, s); return;
     }
   } else {
     launch_mfma_cfg<MT, 2, 4>(d, K, f16, s);
