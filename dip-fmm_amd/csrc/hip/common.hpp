@@ -137,6 +137,12 @@ struct DevicePlan {
   // with NP = n rounded up to 8 (rows n..NP-1 zero), levels 0..L.
   void* m2m_dense = nullptr;
   void* l2l_dense = nullptr;
+  // Harmonic-subspace shift operators (same layout, nc rows/columns, NP =
+  // nc rounded up to 8): Mc_parent = sum_c A_c Mc_child and Lcc_child +=
+  // B_c Lcc_parent, with the level scalings folded in.  Null: the upward and
+  // downward passes run on the full n coefficients.
+  void* m2m_dense_c = nullptr;
+  void* l2l_dense_c = nullptr;
   // state (capacity K)
   int kcap = 0;
   void* moments_in = nullptr;  // host-path staging: K x ns x 3
@@ -208,12 +214,15 @@ template <class S>
 void launch_zero(S* ptr, size_t count, hipStream_t s);
 
 void launch_m2l_split(const DevicePlan& d, int K, int mode, hipStream_t s);  // all levels 2..L
-// Harmonic-subspace transforms around the M2L, levels 2..L: M -> Mc (or the
-// MFMA split layout when `split`), and Lcc -> Lc.
+// Harmonic-subspace transforms around the M2L, levels first_level..L (at
+// least 2): M -> Mc (or the MFMA split layout when `split`), and Lcc -> Lc.
 template <class S>
-void launch_m2l_compress(const DevicePlan& d, int K, bool split, hipStream_t s);
+void launch_m2l_compress(const DevicePlan& d, int K, bool split, int first_level, hipStream_t s);
 template <class S>
-void launch_m2l_expand(const DevicePlan& d, int K, hipStream_t s);
+void launch_m2l_expand(const DevicePlan& d, int K, int first_level, hipStream_t s);
+// bf16 hi/lo split of the compressed multipoles (levels 2..L) into the MFMA
+// operand layout, for plans whose upper levels are formed in compressed form.
+void launch_m2l_bf16_split(const DevicePlan& d, int K, hipStream_t s);
 void launch_m2l_mfma(const DevicePlan& d, int K, int mode, hipStream_t s);   // all levels 2..L
 void prepare_mfma_operators(DevicePlan& d, int n, const std::vector<double>& matrices, int classes,
                             const std::vector<double>& local_scaling, std::vector<void*>& allocations, size_t& bytes);

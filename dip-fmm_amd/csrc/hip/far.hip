@@ -869,11 +869,11 @@ void launch_l2p_unsort(const DevicePlan& d, S* fields, S* potentials, int K, hip
 }
 
 template <class S>
-void launch_m2l_compress(const DevicePlan& d, int K, bool split, hipStream_t s) {
+void launch_m2l_compress(const DevicePlan& d, int K, bool split, int first_level, hipStream_t s) {
   const int NPc = round_up(d.nc, 16);
   const int kc = state_chunk(K, [&](int c) { return size_t(d.n) * round_up(c, kRK) * sizeof(S); });
   const int KP = round_up(kc, kRK);
-  for (int level = 2; level <= d.L; ++level) {
+  for (int level = std::max(first_level, 2); level <= d.L; ++level) {
     const dim3 grid(unsigned(int64_t(1) << (3 * level)), unsigned((K + kc - 1) / kc));
     const unsigned threads = tile_threads(NPc / kRA, KP);
     const size_t lds = size_t(d.n) * KP * sizeof(S);
@@ -891,11 +891,11 @@ void launch_m2l_compress(const DevicePlan& d, int K, bool split, hipStream_t s) 
 }
 
 template <class S>
-void launch_m2l_expand(const DevicePlan& d, int K, hipStream_t s) {
+void launch_m2l_expand(const DevicePlan& d, int K, int first_level, hipStream_t s) {
   const int NP = round_up(d.n, kRA);
   const int kc = state_chunk(K, [&](int c) { return size_t(d.nc) * round_up(c, kRK) * sizeof(S); });
   const int KP = round_up(kc, kRK);
-  for (int level = 2; level <= d.L; ++level) {
+  for (int level = std::max(first_level, 2); level <= d.L; ++level) {
     const dim3 grid(unsigned(int64_t(1) << (3 * level)), unsigned((K + kc - 1) / kc));
     expand_kernel<S><<<grid, tile_threads(NP / kRA, KP), size_t(d.nc) * KP * sizeof(S), s>>>(d, level, K, kc, KP);
     HIPFMM_CHECK(hipGetLastError());
@@ -916,8 +916,8 @@ void launch_zero(S* ptr, size_t count, hipStream_t s) {
   template void launch_m2m<S>(const DevicePlan&, int, int, hipStream_t);           \
   template void launch_l2l<S>(const DevicePlan&, int, int, hipStream_t);           \
   template void launch_l2p_unsort<S>(const DevicePlan&, S*, S*, int, hipStream_t);     \
-  template void launch_m2l_compress<S>(const DevicePlan&, int, bool, hipStream_t);  \
-  template void launch_m2l_expand<S>(const DevicePlan&, int, hipStream_t);         \
+  template void launch_m2l_compress<S>(const DevicePlan&, int, bool, int, hipStream_t); \
+  template void launch_m2l_expand<S>(const DevicePlan&, int, int, hipStream_t);    \
   template void launch_zero<S>(S*, size_t, hipStream_t);
 HIPFMM_INSTANTIATE(float)
 HIPFMM_INSTANTIATE(double)

@@ -671,7 +671,37 @@ __global__ void __launch_bounds__(256) split_f16_kernel(const DevicePlan d, int 
   }
 }
 
+// bf16 hi/lo split of Mc into the MFMA B layout mt[node][ks][K][16] (levels
+// 2..L); each thread packs 8 consecutive coefficients of one column.
+__global__ void __launch_bounds__(256) split_bf16_kernel(const DevicePlan d, int K) {
+  const int nc = d.nc, AP = d.apad, KS = AP / 16;
+  const int64_t node = level_offset(2) + blockIdx.x;
+  const float* Mc = static_cast<const float*>(d.Mc) + node * int64_t(nc) * K;
+  for (int w = threadIdx.x; w < K * (AP / 8); w += blockDim.x) {
+    const int k = w % K, a0 = (w / K) * 8;
+    uint32_t hi[4], lo[4];
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+      const float x0 = a0 + e < nc ? Mc[int64_t(a0 + e) * K + k] : 0.0f;
+      const float x1 = a0 + e + 1 < nc ? Mc[int64_t(a0 + e + 1) * K + k] : 0.0f;
+      const uint16_t h0 = bf16_bits(x0), h1 = bf16_bits(x1);
+      hi[e / 2] = uint32_t(h0) | (uint32_t(h1) << 16);
+      lo[e / 2] = uint32_t(bf16_bits(x0 - bf16_value(h0))) | (uint32_t(bf16_bits(x1 - bf16_value(h1))) << 16);
+    }
+    const int64_t at = (((node * KS + a0 / 16) * K + k) * 16 + a0 % 16) / 8;  // uint4 index
+    static_cast<uint4*>(d.mt_hi)[at] = uint4{hi[0], hi[1], hi[2], hi[3]};
+    static_cast<uint4*>(d.mt_lo)[at] = uint4{lo[0], lo[1], lo[2], lo[3]};
+  }
+}
+
 }  // namespace
+
+void launch_m2l_bf16_split(const DevicePlan& d, int K, hipStream_t s) {
+  if (d.L < 2) return;
+  const unsigned boxes = unsigned(d.nodes - level_offset(2));
+  split_bf16_kernel<<<boxes, 256, 0, s>>>(d, K);
+  HIPFMM_CHECK(hipGetLastError());
+}
 
 void launch_m2l_fp16_split(const DevicePlan& d, int K, hipStream_t s) {
   if (d.L < 2) return;

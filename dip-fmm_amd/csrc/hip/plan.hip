@@ -421,6 +421,80 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
     d.exp_op = upload(P, expd.data(), expd.size());
     d.unit_scale = upload(P, ones.data(), ones.size());
     P->stats.m2l_reconstruction_error = hb.max_reconstruction_error;
+    // FP32 plans run the upward and downward passes in the harmonic subspace
+    // too.  A multipole with no harmonic part has no field, and neither has
+    // its translate, so P M2M = A P with A = P ms_p S ms_c^-1 P^T; the Taylor
+    // polynomial of a harmonic local stays harmonic under L2L, so
+    // L2L Q = Q B with B = Q^T ls_c^-1 S ls_p Q (ms/ls per-degree, commuting
+    // with the block-diagonal bases).  Both identities are checked on every
+    // operator; if one fails the passes stay on the full coefficients.
+    if (sizeof(S) == 4 && L >= 3 && desc.m2m_operators && desc.l2l_operators) {
+      const int NPc8 = (nc + 7) / 8 * 8;
+      std::vector<S> am(size_t(L + 1) * 8 * nc * NPc8, S(0)), bl(am.size(), S(0));
+      std::vector<double> X(size_t(nc) * n), Y(size_t(n) * nc);
+      double leak = 0.0;
+      for (int l = 1; l <= L; ++l)
+        for (int c = 0; c < 8; ++c) {
+          const double* Sm = desc.m2m_operators + (size_t(l - 1) * 8 + c) * n * n;  // row-major out x in
+          const double* Sl = desc.l2l_operators + (size_t(l - 1) * 8 + c) * n * n;
+          const double* msp = desc.m2l_multipole_scaling + size_t(l - 1) * n;
+          const double* msc = desc.m2l_multipole_scaling + size_t(l) * n;
+          const double* lsp = desc.m2l_local_scaling + size_t(l - 1) * n;
+          const double* lsc = desc.m2l_local_scaling + size_t(l) * n;
+          // X = P ms_p S ms_c^-1 (nc x n), then A = X P^T; leak = X (I - P^T P).
+          for (int r = 0; r < nc; ++r)
+            for (int b = 0; b < n; ++b) {
+              double acc = 0.0;
+              for (int a = 0; a < n; ++a) acc += hb.P[size_t(r) * n + a] * msp[a] * Sm[size_t(a) * n + b];
+              X[size_t(r) * n + b] = acc / msc[b];
+            }
+          // Y = ls_c^-1 S ls_p Q (n x nc), then B = Q^T Y; leak = (I - Q Q^T) Y.
+          for (int a = 0; a < n; ++a)
+            for (int r = 0; r < nc; ++r) {
+              double acc = 0.0;
+              for (int b = 0; b < n; ++b) acc += Sl[size_t(a) * n + b] * lsp[b] * hb.Q[size_t(b) * nc + r];
+              Y[size_t(a) * nc + r] = acc / lsc[a];
+            }
+          double xmax = 0.0, ymax = 0.0, xl = 0.0, yl = 0.0;
+          S* ao = am.data() + (size_t(l) * 8 + c) * nc * NPc8;
+          S* bo = bl.data() + (size_t(l) * 8 + c) * nc * NPc8;
+          std::vector<double> A(size_t(nc) * nc), B(size_t(nc) * nc);
+          for (int r = 0; r < nc; ++r)
+            for (int q = 0; q < nc; ++q) {
+              double av = 0.0, bv = 0.0;
+              for (int a = 0; a < n; ++a) {
+                av += X[size_t(r) * n + a] * hb.P[size_t(q) * n + a];
+                bv += hb.Q[size_t(a) * nc + r] * Y[size_t(a) * nc + q];
+              }
+              A[size_t(r) * nc + q] = av;
+              B[size_t(r) * nc + q] = bv;
+              ao[size_t(q) * NPc8 + r] = S(av);  // transposed: [column q][row r]
+              bo[size_t(q) * NPc8 + r] = S(bv);
+            }
+          // Leaks: X - A P and Y - Q B.
+          for (int r = 0; r < nc; ++r)
+            for (int b = 0; b < n; ++b) {
+              double proj = 0.0;
+              for (int q = 0; q < nc; ++q) proj += A[size_t(r) * nc + q] * hb.P[size_t(q) * n + b];
+              xmax = std::max(xmax, std::fabs(X[size_t(r) * n + b]));
+              xl = std::max(xl, std::fabs(X[size_t(r) * n + b] - proj));
+            }
+          for (int a = 0; a < n; ++a)
+            for (int r = 0; r < nc; ++r) {
+              double proj = 0.0;
+              for (int q = 0; q < nc; ++q) proj += hb.Q[size_t(a) * nc + q] * B[size_t(q) * nc + r];
+              ymax = std::max(ymax, std::fabs(Y[size_t(a) * nc + r]));
+              yl = std::max(yl, std::fabs(Y[size_t(a) * nc + r] - proj));
+            }
+          if (xmax > 0.0) leak = std::max(leak, xl / xmax);
+          if (ymax > 0.0) leak = std::max(leak, yl / ymax);
+        }
+      if (leak <= 1e-10) {
+        d.m2m_dense_c = upload(P, am.data(), am.size());
+        d.l2l_dense_c = upload(P, bl.data(), bl.size());
+      }
+      P->stats.harmonic_pass_leak = leak;
+    }
   }
   d.m2l_T = upload(P, T.data(), T.size());
   d.m2l_slot = upload(P, slot.data(), slot.size());
@@ -757,7 +831,21 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   HIPFMM_CHECK(hipEventRecord(e[kEvNear1], P->s_near));
   hipfmm::launch_p2m<S>(d, K, s);
   mark(kEvP2M, s);
-  for (int level = d.L - 1; level >= 2; --level) hipfmm::launch_m2m<S>(d, level, K, s);
+  // Harmonic passes: leaf multipoles compressed once, M2M and (below) L2L on
+  // the compressed coefficients, the expansion at the leaf level only.
+  const bool harmonic = d.m2m_dense_c != nullptr && d.nc > 0 && d.L >= 3;
+  DevicePlan dc = d;
+  if (harmonic) {
+    dc.n = d.nc;
+    dc.M = d.Mc;
+    dc.Lc = d.Lcc;
+    dc.m2m_dense = d.m2m_dense_c;
+    dc.l2l_dense = d.l2l_dense_c;
+    hipfmm::launch_m2l_compress<S>(d, K, false, d.L, s);
+    for (int level = d.L - 1; level >= 2; --level) hipfmm::launch_m2m<S>(dc, level, K, s);
+  } else {
+    for (int level = d.L - 1; level >= 2; --level) hipfmm::launch_m2m<S>(d, level, K, s);
+  }
   // Levels 0 and 1 hold no list-2 work; keep their coefficients zero.
   const size_t top = size_t(std::min<int64_t>(d.nodes, 9)) * d.n * K;
   hipfmm::launch_zero<S>(static_cast<S*>(d.M), top, s);
@@ -776,8 +864,15 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     // bf16 operands come straight out of the compression; FP16 operands need
     // the level maxima of the compressed multipoles first.
     const bool fp16 = mfma && P->m2l_mode == HIPFMM_M2L_MFMA_FP16X3;
-    hipfmm::launch_m2l_compress<S>(d, K, mfma && !fp16, s);
-    if (fp16) hipfmm::launch_m2l_fp16_split(d, K, s);
+    if (harmonic) {
+      if (fp16)
+        hipfmm::launch_m2l_fp16_split(d, K, s);
+      else if (mfma)
+        hipfmm::launch_m2l_bf16_split(d, K, s);
+    } else {
+      hipfmm::launch_m2l_compress<S>(d, K, mfma && !fp16, 2, s);
+      if (fp16) hipfmm::launch_m2l_fp16_split(d, K, s);
+    }
   } else if (mfma) {
     hipfmm::launch_m2l_split(d, K, P->m2l_mode, s);
   }
@@ -797,9 +892,14 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     P->m2l_ring_pending[ring_slot] = true;
     ++P->m2l_launch_count;
   }
-  if (d.nc > 0 && d.L >= 2) hipfmm::launch_m2l_expand<S>(d, K, s);
+  if (d.nc > 0 && d.L >= 2 && !harmonic) hipfmm::launch_m2l_expand<S>(d, K, 2, s);
   mark(kEvM2L, s);
-  for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(d, level, K, s);
+  if (harmonic) {
+    for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(dc, level, K, s);
+    hipfmm::launch_m2l_expand<S>(d, K, d.L, s);
+  } else {
+    for (int level = 3; level <= d.L; ++level) hipfmm::launch_l2l<S>(d, level, K, s);
+  }
   mark(kEvL2L, s);
   HIPFMM_CHECK(hipStreamWaitEvent(s, e[kEvNear1], 0));
   hipfmm::launch_l2p_unsort<S>(d, out, pot, K, s);
@@ -1058,6 +1158,8 @@ int hipfmm_plan_coefficients(hipfmm_plan* plan, int which, int node, int state, 
     const DevicePlan& d = plan->d;
     require(plan->last_K > 0, "no evaluation has run");
     require(node >= 0 && node < d.nodes && state >= 0 && state < plan->last_K, "node or state out of range");
+    require(d.m2m_dense_c == nullptr || node >= hipfmm::level_offset(d.L),
+            "this plan forms the expansions above the leaf level in harmonic-subspace form only");
     DeviceGuard guard(plan->device);
     HIPFMM_CHECK(hipStreamSynchronize(plan->s_far));
     const int K = plan->last_K;

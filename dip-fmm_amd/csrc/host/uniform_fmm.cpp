@@ -652,6 +652,7 @@ HipPlanStatistics UniformFmm::hip_plan_statistics() const {
   o.m2l_leaf_interaction_count = s.m2l_leaf_interaction_count;
   o.m2l_operand_size = s.m2l_operand_size;
   o.m2l_reconstruction_error = s.m2l_reconstruction_error;
+  o.harmonic_pass_leak = s.harmonic_pass_leak;
   o.setup_seconds = setup_seconds_;
   o.m2l_tile_shape = s.m2l_tile_shape;
   o.p2p_kernel_variant = s.p2p_kernel_variant;

@@ -142,6 +142,7 @@ struct HipPlanStatistics {
       static_upload_count{0}, batch_capacity{0}, m2l_leaf_interaction_count{0};
   std::size_t m2l_operand_size{0};       // n, or (p+1)^2 in harmonic-subspace form
   double m2l_reconstruction_error{0.0};  // of the harmonic-subspace factorisation
+  double harmonic_pass_leak{0.0};        // M2M/L2L outside the harmonic subspace (0: full passes)
   double setup_seconds{0.0};             // host plan construction + upload
   std::size_t m2l_tile_shape{0}, p2p_kernel_variant{0};  // launch choices (autotune)
   double autotune_seconds{0.0};

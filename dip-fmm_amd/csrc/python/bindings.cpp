@@ -485,6 +485,7 @@ m.def("hip_full_available", &hip_full_available);
                                d["m2l_leaf_interaction_count"] = s.m2l_leaf_interaction_count;
                                d["m2l_operand_size"] = s.m2l_operand_size;
                                d["m2l_reconstruction_error"] = s.m2l_reconstruction_error;
+                               d["harmonic_pass_leak"] = s.harmonic_pass_leak;
                                d["setup_seconds"] = s.setup_seconds;
                                d["m2l_tile_shape"] = s.m2l_tile_shape;
                                d["p2p_kernel_variant"] = s.p2p_kernel_variant;

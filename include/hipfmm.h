@@ -135,6 +135,11 @@ typedef struct hipfmm_stats {
    * (T_o = Q C_o P; no counterpart in the reference, which always uses n). */
   uint64_t m2l_operand_size;
   double m2l_reconstruction_error; /* max |T_o - Q C_o P| / max |T_o| over classes (0 if unused) */
+  /* FP32 plans with the harmonic form run M2M and L2L on the compressed
+   * coefficients; this is the largest relative part of an M2M or L2L
+   * operator outside the harmonic subspace (checked <= 1e-10 at setup; 0 if
+   * the passes run on the full coefficients). */
+  double harmonic_pass_leak;
   /* Launch choices in use (hipfmm_plan_autotune; no counterpart in the
    * reference): the matrix-core M2L tile shape code, 10 * column tiles per
    * wave + ring stages (0 = built-in default), and the near-field kernel at

@@ -428,3 +428,28 @@ def test_batched_expansions_match_oracle_fp32(hip):
                 assert not np.any(got)
                 continue
             assert np.abs(got - ref).max() <= 2e-5 * scale, (k, node)
+
+
+def test_harmonic_passes_on_fp32_plans(hip):
+    # FP32 plans of depth >= 3 run M2M and L2L on the (p+1)^2 compressed
+    # coefficients (the operators' leak out of the harmonic subspace is
+    # checked at setup); results stay within the FP32 bar of the FP64 FMM,
+    # whose passes run on the full coefficients, and non-leaf expansions are
+    # refused by the accessor.
+    rng = np.random.default_rng(999)
+    n = 20000
+    x = rng.uniform(0, 1, size=(n, 3))
+    m = rng.normal(size=(n, 3))
+    announce("harmonic passes")
+    f = make(hip, x, x, 8, 4, 32, ids=np.arange(n), mode=hip.HipM2LMode.MFMA_BF16X3)
+    st = f.hip_plan_statistics
+    assert 0.0 <= st["harmonic_pass_leak"] <= 1e-10
+    h = f.evaluate(m)["H"]
+    ref = O.CpuFmm(x, x, 8, 4, 64, fixed_ids=np.arange(n)).evaluate(m)["H"]
+    e = rel_l2(h, ref)
+    print(f"  leak {st['harmonic_pass_leak']:.2e}; rel L2 vs FP64 FMM {e:.3e}", flush=True)
+    assert e <= 1e-5
+    f.multipole(585)  # a leaf of depth 4 (level offset 585)
+    with pytest.raises(ValueError, match="harmonic-subspace form"):
+        f.multipole(9)
+    assert make(hip, x, x, 8, 4, 64, ids=np.arange(n)).hip_plan_statistics["harmonic_pass_leak"] == 0.0
