@@ -190,7 +190,7 @@ struct DevicePlan {
 
 // MFMA M2L tile shapes of the harmonic form (<= 3 row tiles): the code is
 // 10 * (column tiles per wave) + (ring stages).
-constexpr int kM2LShapes[] = {43, 53, 24, 34, 26};
+constexpr int kM2LShapes[] = {23, 13, 43, 53, 24, 34, 26};
 
 // Kernel launchers (one translation unit each).
 template <class S>

@@ -559,11 +559,13 @@ void launch_mfma_cfg(const DevicePlan& d, int K, bool f16, hipStream_t s) {
 }
 
 // Tile shape per row-tile count MT: NB 32-column tiles per wave (NB * MT *
-// 16 accumulator registers; at MT <= 3 the default 5 x 3 keeps its 240 in
-// AGPRs) and the ring depth that fits LDS (160 KiB per workgroup).  The
-// shape only regroups whole output columns, so every shape gives bitwise the
-// same locals.  HIPFMM_M2L_MFMA_SHAPE=NBxS (2x4, 3x4, 4x3, 5x3, 2x6)
-// overrides the choice among the compiled shapes, for measurement.
+// 16 accumulator registers) and the ring depth that fits LDS (160 KiB per
+// CU).  At MT <= 3 the default 2 x 3 takes 77 KB, so two workgroups share a
+// CU and one's barrier wait overlaps the other's products (81% MFMA-busy,
+// against 67% for 5 x 3 at one workgroup per CU).  The shape only regroups
+// whole output columns, so every shape gives bitwise the same locals.
+// HIPFMM_M2L_MFMA_SHAPE=NBxS (1x3, 2x3, 2x4, 3x4, 4x3, 5x3, 2x6) overrides
+// the choice among the compiled shapes, for measurement.
 inline int mfma_shape_override() {
   static const int v = [] {
     const char* e = std::getenv("HIPFMM_M2L_MFMA_SHAPE");
@@ -573,6 +575,8 @@ inline int mfma_shape_override() {
     if (std::string(e) == "4x3") return 43;
     if (std::string(e) == "2x6") return 26;
     if (std::string(e) == "5x3") return 53;
+    if (std::string(e) == "2x3") return 23;
+    if (std::string(e) == "1x3") return 13;
     return 0;
   }();
   return v;
@@ -586,7 +590,10 @@ void launch_mfma_t(const DevicePlan& d, int K, bool f16, hipStream_t s) {
       case 34: launch_mfma_cfg<MT, 3, 4>(d, K, f16, s); return;
       case 26: launch_mfma_cfg<MT, 2, 6>(d, K, f16, s); return;
       case 43: launch_mfma_cfg<MT, 4, 3>(d, K, f16, s); return;
-      default: launch_mfma_cfg<MT, 5, 3>(d, K, f16, s); return;
+      case 23: launch_mfma_cfg<MT, 2, 3>(d, K, f16, s); return;
+      case 53: launch_mfma_cfg<MT, 5, 3>(d, K, f16, s); return;
+      case 13: launch_mfma_cfg<MT, 1, 3>(d, K, f16, s); return;
+      default: launch_mfma_cfg<MT, 2, 3>(d, K, f16, s); return;
     }
   } else {
     launch_mfma_cfg<MT, 2, 4>(d, K, f16, s);

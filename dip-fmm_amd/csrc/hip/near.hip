@@ -344,36 +344,55 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // and padding contribute zero; coincident pairs give NaN (r = 0).  Blocked
 // summation per 32 sources as in the 32 x 32 kernel.  |r|^-1 is the hardware
 // reciprocal square root (1 ulp).
-// One step's operands of the targets-as-rows kernel for this lane's pair
-// (row target, source s): the 6 distinct entries of D (d00 d01 d02 d11 d12
-// d22), the potential row f r, and the moments m[s][c'][k] of both 16-state
-// halves.  Sources at or past `end` (and the identity pair) give zeros; the
-// load index is clamped to `first` so every access stays in the leaf.
+// One step of the targets-as-rows kernel for this lane's pair (row target,
+// source s), in two stages so the loads can run two steps ahead of the
+// products: the raw loads (source position, moments m[s][c'][k] of both
+// 16-state halves; the index is clamped to `first` so every load is in the
+// leaf and unconditional), then the operands -- the 6 distinct entries of D
+// (d00 d01 d02 d11 d12 d22), the potential row f r, and the masked moments.
+// Sources at or past `end` and the identity pair give zeros.
+struct P2PRaw {
+  PairPos<float> sp;
+  float b[3][2];
+  int s;
+};
+
 struct P2POperands {
   float d[6];
   float p[3];
   float b[3][2];
 };
 
-template <int kComps>
-__device__ __forceinline__ P2POperands p2p_operands(const DevicePlan& d, const PairPos<float>& tp,
-                                                    const float* __restrict__ sm, bool row_ok, int self, int s,
-                                                    int end, int first, int K, int kcol) {
-  P2POperands o;
-  const bool src_ok = s < end;
-  const int sl = src_ok ? s : first;
-  float rx, ry, rz;
-  tp.diff(PairPos<float>::load(d.src_pos, d.ns, sl), rx, ry, rz);
+__device__ __forceinline__ P2PRaw p2p_load(const DevicePlan& d, const float* __restrict__ sm, int s, int end,
+                                           int first, int K, int kcol) {
+  P2PRaw r;
+  const int sl = s < end ? s : first;
+  r.sp = PairPos<float>::load(d.src_pos, d.ns, sl);
   const float* ms = sm + int64_t(sl) * 3 * K + kcol;
 #pragma unroll
   for (int cp = 0; cp < 3; ++cp)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) o.b[cp][h] = src_ok ? ms[cp * K + 16 * h] : 0.0f;
+    for (int h = 0; h < 2; ++h) r.b[cp][h] = ms[cp * K + 16 * h];
+  r.s = s;
+  return r;
+}
+
+template <int kComps>
+__device__ __forceinline__ P2POperands p2p_finish(const P2PRaw& r, const PairPos<float>& tp, bool row_ok, int self,
+                                                  int end) {
+  P2POperands o;
+  const bool src_ok = r.s < end;
+  float rx, ry, rz;
+  tp.diff(r.sp, rx, ry, rz);
+#pragma unroll
+  for (int cp = 0; cp < 3; ++cp)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) o.b[cp][h] = src_ok ? r.b[cp][h] : 0.0f;
   const float r2 = rx * rx + ry * ry + rz * rz;
   const float rinv = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32 (1 ulp); r = 0 gives inf, hence NaN below
   const float f = rinv * rinv * rinv * 0.079577471545947667884441881686257181f;
   const float ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
-  const bool use = row_ok & src_ok & (s != self);
+  const bool use = row_ok & src_ok & (r.s != self);
   const float f3 = 3.0f * f;
   o.d[0] = use ? f3 * ux * ux - f : 0.0f;
   o.d[1] = use ? f3 * ux * uy : 0.0f;
@@ -428,12 +447,19 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const Device
           for (int c = 0; c < kComps; ++c)
 #pragma unroll
             for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-          // Software pipelined: the loads and pair geometry of step j0 + 4
-          // are issued before the products of step j0, so they overlap the
-          // matrix pipe.
-          P2POperands cur = p2p_operands<kComps>(d, tp, sm, row_ok, self, jb + q, je, sr.x, K, kc0 + col);
+          // Software pipelined: the loads of step j0 + 8 and the pair
+          // geometry of step j0 + 4 (loaded one iteration earlier) are
+          // issued before the products of step j0.
+          const P2PRaw r0 = p2p_load(d, sm, jb + q, je, sr.x, K, kc0 + col);
+          P2PRaw r1 = p2p_load(d, sm, jb + 4 + q, je, sr.x, K, kc0 + col);
+          P2POperands cur = p2p_finish<kComps>(r0, tp, row_ok, self, je);
+          // Unrolled by two so the rotation r1 <- r2 renames registers
+          // instead of copying (a copy would wait for the fresh loads).
+#pragma unroll 2
           for (int j0 = jb; j0 < je; j0 += 4) {
-            const P2POperands nxt = p2p_operands<kComps>(d, tp, sm, row_ok, self, j0 + 4 + q, je, sr.x, K, kc0 + col);
+            const P2PRaw r2 = p2p_load(d, sm, j0 + 8 + q, je, sr.x, K, kc0 + col);
+            const P2POperands nxt = p2p_finish<kComps>(r1, tp, row_ok, self, je);
+            r1 = r2;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
               part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[0], cur.b[0][h], part[0][h], 0, 0, 0);

@@ -253,7 +253,7 @@ int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode);
 int hipfmm_plan_autotune(hipfmm_plan* plan, int batch);
 
 /* Set the launch choices directly (values as reported in hipfmm_stats: a
- * tile shape code 43, 53, 24, 34 or 26, or 0 for the default; a near-field
+ * tile shape code 23, 13, 43, 53, 24, 34 or 26, or 0 for the default; a near-field
  * variant 0..3), e.g. to reuse an earlier autotune result.  No counterpart in
  * the reference. */
 int hipfmm_plan_set_launch_choices(hipfmm_plan* plan, int m2l_tile_shape, int p2p_kernel_variant);

@@ -42,7 +42,7 @@ def announce(*what):
     print("case:", *what, flush=True)
 
 
-SHAPES = {43, 53, 24, 34, 26}
+SHAPES = {23, 13, 43, 53, 24, 34, 26}
 
 
 @pytest.mark.parametrize("mode", ["MFMA_BF16X3", "MFMA_FP16X3"])
