@@ -182,6 +182,7 @@ struct DevicePlan {
   unsigned* lvl_max_bits = nullptr;  // L+1: largest |Mc / sigma| per level (float bits)
   const void* zero_block = nullptr;  // 1 KiB of zeros: the operand of sources outside the grid
   int apad = 0, bpad = 0;
+  int mfma_depth_steps = 0;  // 16-deep steps the MFMA M2L reads (<= apad / 16; 0 = apad / 16)
   // Launch choices (set by the autotuner; 0 = built-in default).
   int m2l_shape = 0;    // MFMA M2L tile shape code (kM2LShapes)
   int p2p_variant = 0;  // near field at K % 32 == 0 (FP32): 1 vector ALU, 2 matrix cores with
@@ -236,6 +237,8 @@ void launch_m2l_fp16_split(const DevicePlan& d, int K, hipStream_t s);
 // T_o = Q C_o P with orthonormal block-diagonal bases; nc = 0 when the
 // matrices are not of that form.
 struct HarmonicBasis {
+  // Compressed index r holds the harmonic basis function (r + 1) mod nc of
+  // the degree-major order (degree j at j^2 .. (j+1)^2 - 1): degree 0 last.
   int nc = 0;              // (p+1)^2, or 0
   std::vector<double> P;   // nc x n row-major: compressed multipole = P * multipole
   std::vector<double> Q;   // n x nc row-major: local = Q * compressed local

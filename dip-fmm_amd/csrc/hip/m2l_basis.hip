@@ -171,10 +171,25 @@ HarmonicBasis harmonic_m2l_basis(int p, const double* T, int classes) {
     if (tmax > 0.0) worst = std::max(worst, err / tmax);
   }
   if (!(worst <= 1e-11)) return h;
+  // Degree 0 goes last: compressed index r holds harmonic row (r + 1) mod nc.
+  // Multipoles of dipole sources have no monopole (P2M writes none, M2M maps
+  // it only to itself), so the last compressed multipole is exactly zero and
+  // the matrix-core M2L can stop its depth at nc - 1.
+  std::vector<double> Pr(P.size()), Qr(Q.size()), Cr(C.size());
+  for (int r = 0; r < nc; ++r) {
+    const int o = (r + 1) % nc;
+    for (int a = 0; a < n; ++a) Pr[size_t(r) * n + a] = P[size_t(o) * n + a];
+    for (int b = 0; b < n; ++b) Qr[size_t(b) * nc + r] = Q[size_t(b) * nc + o];
+  }
+  for (int cls = 0; cls < classes; ++cls)
+    for (int a = 0; a < nc; ++a)
+      for (int b = 0; b < nc; ++b)
+        Cr[size_t(cls) * nc * nc + b + size_t(nc) * a] =
+            C[size_t(cls) * nc * nc + (b + 1) % nc + size_t(nc) * ((a + 1) % nc)];
   h.nc = nc;
-  h.P = std::move(P);
-  h.Q = std::move(Q);
-  h.C = std::move(C);
+  h.P = std::move(Pr);
+  h.Q = std::move(Qr);
+  h.C = std::move(Cr);
   h.max_reconstruction_error = worst;
   return h;
 }

@@ -185,7 +185,7 @@ __device__ __forceinline__ bool source_inside(const Columns<NB>& c, int nb, int 
 }
 
 struct StepGeometry {
-  int K, KS, level, nbox, nsteps;
+  int K, KS, KD, level, nbox, nsteps;  // KS: 16-deep slices per box (stride); KD: slices multiplied
   int64_t lvl0;
 };
 
@@ -338,6 +338,7 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
   StepGeometry sg;
   sg.K = K;
   sg.KS = d.apad / 16;
+  sg.KD = d.mfma_depth_steps > 0 ? d.mfma_depth_steps : sg.KS;
   sg.level = level;
   sg.nbox = 1 << level;
   sg.lvl0 = level_offset(level);
@@ -415,7 +416,7 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
   }
   __syncthreads();
   const int nact = active_count;
-  sg.nsteps = sg.KS * nact;
+  sg.nsteps = sg.KD * nact;
   if constexpr (kUniformBox) {
     // Step tables of the used offsets.
     for (int e = threadIdx.x; e < nact * (W * NB + 1); e += W * 64) {
@@ -455,7 +456,7 @@ __global__ void __launch_bounds__(W * 64, 1) m2l_mfma_kernel(const DevicePlan d,
     // is fetched again into a slot nobody reads.
     int iks = 0, ioi = 0;
     auto advance = [&]() {  // branch-free: selects only
-      const bool last = (iks == sg.KS - 1) & (ioi == nact - 1);
+      const bool last = (iks == sg.KD - 1) & (ioi == nact - 1);
       const bool wrap = ioi + 1 == nact;
       const int next_oi = wrap ? 0 : ioi + 1;
       const int next_ks = iks + (wrap ? 1 : 0);
@@ -724,10 +725,11 @@ void prepare_mfma_fp16_operators(DevicePlan& d, int n, const std::vector<double>
                                  std::vector<void*>& allocations, size_t& bytes) {
   const int MT = d.bpad / 32, KS = d.apad / 16, D = order + 1;
   if (n != D * D) throw invalid_argument("the FP16 M2L mode needs class matrices of harmonic form");
-  // Degree of each harmonic row/column (rows j^2 .. (j+1)^2 - 1 have degree j).
+  // Degree of each compressed row/column: index r is harmonic row
+  // (r + 1) mod n of the degree-major order (degree 0 last).
   std::vector<int> deg(n);
   for (int j = 0; j < D; ++j)
-    for (int r = j * j; r < (j + 1) * (j + 1); ++r) deg[r] = j;
+    for (int r = j * j; r < (j + 1) * (j + 1); ++r) deg[(r + n - 1) % n] = j;
   // log2 of the largest entry of each degree block (j, k) over all classes.
   std::vector<double> lt(size_t(D) * D, 0.0), tab(size_t(D) * D, 0.0);
   for (int c = 0; c < classes; ++c)

@@ -593,6 +593,9 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
         const std::vector<double> ones(size_t(L + 1) * d.nc, 1.0);
         hipfmm::prepare_mfma_operators(d, d.nc, hb.C, desc.m2l_class_count, ones, P->persistent, bytes);
         hipfmm::prepare_mfma_fp16_operators(d, d.nc, hb.C, desc.m2l_class_count, p, P->persistent, bytes);
+        // The last compressed multipole (degree 0) is zero for dipole
+        // sources: the depth stops before it.
+        d.mfma_depth_steps = (d.nc - 1 + 15) / 16;
       } else {
         std::vector<double> mats(desc.m2l_matrices, desc.m2l_matrices + size_t(desc.m2l_class_count) * nn);
         std::vector<double> lsd(desc.m2l_local_scaling, desc.m2l_local_scaling + size_t(L + 1) * n);
