@@ -619,11 +619,17 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
       const bool ok = sj < cnt;
       const float* ps = pw + (ok ? sj : 0) * 3 * PW;
       const float* ms = sm + int64_t(j0 + (ok ? sj : 0)) * 3 * K + kc0 + col;
+      // Unconditional loads (clamped index), masked afterwards: no branch
+      // per load.
       float b[3][2];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int h = 0; h < 2; ++h) b[c][h] = ok ? ms[c * K + 16 * h] : 0.0f;
+        for (int h = 0; h < 2; ++h) b[c][h] = ms[c * K + 16 * h];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) b[c][h] = ok ? b[c][h] : 0.0f;
 #pragma unroll
       for (int j = 0; j < kTW; ++j) {
         if (wave + 4 * j >= NT) break;
@@ -679,6 +685,12 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
   const int wave = int(threadIdx.x) >> 6, lane = threadIdx.x & 63;
   float* pw = reinterpret_cast<float*>(smem_raw) + wave * (48 * PW);       // [t][axis][i]
   float* mono = reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + wave * (NM * 16);  // [g][t]
+  // Basis tables shared by the workgroup: exponents of the NM monomials and
+  // the raising table g -> g + e_c (so no step waits on a table load).
+  int4* ex_l = reinterpret_cast<int4*>(reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + 4 * NM * 16);
+  int4* up_l = ex_l + NM;
+  for (int i = threadIdx.x; i < NM; i += 256) ex_l[i] = d.p2m_tab[n + i];
+  for (int i = threadIdx.x; i < d.nm; i += 256) up_l[i] = d.l2p_up[i];
   const int item = int(blockIdx.x) * 4 + wave;
   const bool valid = item < d.p2p_tile16_count;
   const int2 tile = valid ? d.p2p_tiles16[item] : int2{0, 0};
@@ -700,10 +712,10 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
       pw[lane * PW + i] = v;
     }
   }
-  __syncthreads();
+  __syncthreads();  // axis powers and basis tables in LDS
   for (int w = lane; w < NM * 16; w += 64) {
     const int g = w >> 4, t = w & 15;
-    const int4 e = d.p2m_tab[n + g];
+    const int4 e = ex_l[g];
     const float* pt = pw + t * 3 * PW;
     mono[w] = pt[e.x] * pt[PW + e.y] * pt[2 * PW + e.z];
   }
@@ -720,30 +732,46 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
     accp[h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   }
   const int nm = d.nm;
-  for (int g0 = 0; g0 < nm; g0 += 4) {
-    const int g = g0 + q;
-    const bool ok = g < nm;
-    const float a = ok ? mono[g * 16 + col] : 0.0f;
-    const int4 up = ok ? d.l2p_up[g] : int4{0, 0, 0, 0};
-    const int rows[3] = {up.x, up.y, up.z};
+  // Loads are unconditional (past-the-end monomials read row 0 and are
+  // masked afterwards), so they do not become branches that each wait; four
+  // steps' loads are in flight together.
+  for (int g00 = 0; g00 < nm; g00 += 16) {
+    float av[4], bv[4][3][2];
+    bool okv[4];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int u = 0; u < 4; ++u) {
+      const int g = g00 + 4 * u + q;
+      okv[u] = g < nm;
+      const int gl = okv[u] ? g : 0;
+      av[u] = okv[u] ? mono[gl * 16 + col] : 0.0f;
+      const int4 up = up_l[gl];
+      const int rows[3] = {up.x, up.y, up.z};
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const float b = ok ? Lb[int64_t(rows[c]) * K + 16 * h] : 0.0f;
-        acc[c][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[c][h], 0, 0, 0);
-      }
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) bv[u][c][h] = Lb[int64_t(rows[c]) * K + 16 * h];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          acc[c][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], okv[u] ? bv[u][c][h] : 0.0f, acc[c][h], 0, 0, 0);
   }
   if constexpr (kPot) {
+#pragma unroll 4
     for (int b0 = 0; b0 < n; b0 += 4) {
       const int b = b0 + q;
       const bool ok = b < n;
-      const float a = ok ? mono[b * 16 + col] : 0.0f;
+      const int bl = ok ? b : 0;
+      const float a = ok ? mono[bl * 16 + col] : 0.0f;
+      float bv[2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const float bv = ok ? Lb[int64_t(b) * K + 16 * h] : 0.0f;
-        accp[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, accp[h], 0, 0, 0);
-      }
+      for (int h = 0; h < 2; ++h) bv[h] = Lb[int64_t(bl) * K + 16 * h];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        accp[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, ok ? bv[h] : 0.0f, accp[h], 0, 0, 0);
     }
   }
   // acc[c][h][r]: target t0 + 4q + r, state kc0 + 16h + col.
@@ -846,7 +874,8 @@ void launch_l2p_unsort(const DevicePlan& d, S* fields, S* potentials, int K, hip
   if (pot && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
   if constexpr (sizeof(S) == 4) {
     const int NM = pot ? d.n : d.nm;
-    const size_t lds = (size_t(4) * 48 * (d.p + 1) + size_t(4) * NM * 16) * sizeof(float);
+    const size_t lds =
+        (size_t(4) * 48 * (d.p + 1) + size_t(4) * NM * 16) * sizeof(float) + size_t(NM + d.nm) * sizeof(int4);
     if (K % 32 == 0 && d.l2p_up != nullptr && d.p2p_tile16_count > 0 && lds <= 65536) {
       const dim3 grid(unsigned((d.p2p_tile16_count + 3) / 4), unsigned(K / 32));
       if (pot)
