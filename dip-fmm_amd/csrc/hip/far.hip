@@ -614,30 +614,38 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
       }
     }
     __syncthreads();
-    for (int s0 = 0; s0 < cnt; s0 += 4) {
-      const int sj = s0 + q;
-      const bool ok = sj < cnt;
-      const float* ps = pw + (ok ? sj : 0) * 3 * PW;
-      const float* ms = sm + int64_t(j0 + (ok ? sj : 0)) * 3 * K + kc0 + col;
-      // Unconditional loads (clamped index), masked afterwards: no branch
-      // per load.
-      float b[3][2];
+    // Two steps (8 sources) per iteration: both steps' loads are issued
+    // before either's products.  Loads are unconditional (clamped index)
+    // and masked afterwards, so they do not become branches that each wait.
+    for (int s00 = 0; s00 < cnt; s00 += 8) {
+      float b[2][3][2];
+      bool okv[2];
+      int sjv[2];
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) b[c][h] = ms[c * K + 16 * h];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) b[c][h] = ok ? b[c][h] : 0.0f;
-#pragma unroll
-      for (int j = 0; j < kTW; ++j) {
-        if (wave + 4 * j >= NT) break;
-        const float a = (ok && row_ok[j]) ? ps[ex[j].x] * ps[PW + ex[j].y] * ps[2 * PW + ex[j].z] : 0.0f;
+      for (int u = 0; u < 2; ++u) {
+        const int sj = s00 + 4 * u + q;
+        okv[u] = sj < cnt;
+        sjv[u] = okv[u] ? sj : 0;
+        const float* ms = sm + int64_t(j0 + sjv[u]) * 3 * K + kc0 + col;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int h = 0; h < 2; ++h) acc[j][c][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[c][h], acc[j][c][h], 0, 0, 0);
+          for (int h = 0; h < 2; ++h) b[u][c][h] = ms[c * K + 16 * h];
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const float* ps = pw + sjv[u] * 3 * PW;
+#pragma unroll
+        for (int j = 0; j < kTW; ++j) {
+          if (wave + 4 * j >= NT) break;
+          const float a = (okv[u] && row_ok[j]) ? ps[ex[j].x] * ps[PW + ex[j].y] * ps[2 * PW + ex[j].z] : 0.0f;
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+              acc[j][c][h] =
+                  __builtin_amdgcn_mfma_f32_16x16x4f32(a, okv[u] ? b[u][c][h] : 0.0f, acc[j][c][h], 0, 0, 0);
+        }
       }
     }
   }

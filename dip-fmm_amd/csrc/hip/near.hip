@@ -345,8 +345,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // summation per 32 sources as in the 32 x 32 kernel.  |r|^-1 is the hardware
 // reciprocal square root (1 ulp).
 // One step of the targets-as-rows kernel for this lane's pair (row target,
-// source s), in two stages so the loads can run two steps ahead of the
-// products: the raw loads (source position, moments m[s][c'][k] of both
+// source s), in two stages so that several steps' loads are in flight
+// before their products: the raw loads (source position, moments m[s][c'][k] of both
 // 16-state halves; the index is clamped to `first` so every load is in the
 // leaf and unconditional), then the operands -- the 6 distinct entries of D
 // (d00 d01 d02 d11 d12 d22), the potential row f r, and the masked moments.
@@ -447,37 +447,33 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const Device
           for (int c = 0; c < kComps; ++c)
 #pragma unroll
             for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-          // Software pipelined: the loads of step j0 + 8 and the pair
-          // geometry of step j0 + 4 (loaded one iteration earlier) are
-          // issued before the products of step j0.
-          const P2PRaw r0 = p2p_load(d, sm, jb + q, je, sr.x, K, kc0 + col);
-          P2PRaw r1 = p2p_load(d, sm, jb + 4 + q, je, sr.x, K, kc0 + col);
-          P2POperands cur = p2p_finish<kComps>(r0, tp, row_ok, self, je);
-          // Unrolled by two so the rotation r1 <- r2 renames registers
-          // instead of copying (a copy would wait for the fresh loads).
-#pragma unroll 2
-          for (int j0 = jb; j0 < je; j0 += 4) {
-            const P2PRaw r2 = p2p_load(d, sm, j0 + 8 + q, je, sr.x, K, kc0 + col);
-            const P2POperands nxt = p2p_finish<kComps>(r1, tp, row_ok, self, je);
-            r1 = r2;
+          // Two steps (8 sources) per iteration: both steps' loads are
+          // issued before either's pair geometry and products.
+          for (int j00 = jb; j00 < je; j00 += 8) {
+            P2PRaw raw[2];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[0], cur.b[0][h], part[0][h], 0, 0, 0);
-              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], cur.b[0][h], part[1][h], 0, 0, 0);
-              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], cur.b[0][h], part[2][h], 0, 0, 0);
-              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], cur.b[1][h], part[0][h], 0, 0, 0);
-              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[3], cur.b[1][h], part[1][h], 0, 0, 0);
-              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], cur.b[1][h], part[2][h], 0, 0, 0);
-              part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], cur.b[2][h], part[0][h], 0, 0, 0);
-              part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], cur.b[2][h], part[1][h], 0, 0, 0);
-              part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[5], cur.b[2][h], part[2][h], 0, 0, 0);
-              if constexpr (kComps == 4) {
-                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[0], cur.b[0][h], part[3][h], 0, 0, 0);
-                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[1], cur.b[1][h], part[3][h], 0, 0, 0);
-                part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[2], cur.b[2][h], part[3][h], 0, 0, 0);
+            for (int u = 0; u < 2; ++u) raw[u] = p2p_load(d, sm, j00 + 4 * u + q, je, sr.x, K, kc0 + col);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const P2POperands cur = p2p_finish<kComps>(raw[u], tp, row_ok, self, je);
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[0], cur.b[0][h], part[0][h], 0, 0, 0);
+                part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], cur.b[0][h], part[1][h], 0, 0, 0);
+                part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], cur.b[0][h], part[2][h], 0, 0, 0);
+                part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], cur.b[1][h], part[0][h], 0, 0, 0);
+                part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[3], cur.b[1][h], part[1][h], 0, 0, 0);
+                part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], cur.b[1][h], part[2][h], 0, 0, 0);
+                part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], cur.b[2][h], part[0][h], 0, 0, 0);
+                part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], cur.b[2][h], part[1][h], 0, 0, 0);
+                part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[5], cur.b[2][h], part[2][h], 0, 0, 0);
+                if constexpr (kComps == 4) {
+                  part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[0], cur.b[0][h], part[3][h], 0, 0, 0);
+                  part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[1], cur.b[1][h], part[3][h], 0, 0, 0);
+                  part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[2], cur.b[2][h], part[3][h], 0, 0, 0);
+                }
               }
             }
-            cur = nxt;
           }
 #pragma unroll
           for (int c = 0; c < kComps; ++c)
