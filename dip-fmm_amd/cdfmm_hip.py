@@ -82,3 +82,63 @@ def dipole_field(fmm, moments):
         raise ValueError("moments must have shape (n_sources, 3) or (batch, n_sources, 3)")
     h = _FieldMap.apply(m, fmm, ns, nt)
     return h[0] if single else h
+
+
+# ---------------------------------------------------------------------------
+# Gradients with respect to positions.  For the loss L = sum_k sum_t
+# g_kt . H_k(x_t) of a plan, dL/dx_t = G_k(t) g_kt (the field gradient G is
+# symmetric) and, since the dipole kernel is even and symmetric,
+# L = sum_k sum_s m_ks . H'_k(x_s) with H' the field of the dipoles g at the
+# targets, so dL/dx_s = G'_k(s) m_ks.  Both are exact derivatives of the
+# plan's own FMM output with the tree held fixed.
+
+_GRAD_PAIRS = ((0, 1, 2), (1, 3, 4), (2, 4, 5))  # rows of G from its six stored entries
+
+
+def _apply_gradient_device(fmm, x, n_out, adjoint):
+    x = x.to(_plan_dtype(fmm)).contiguous()
+    out = torch.empty((x.shape[0], n_out, 6), dtype=x.dtype, device=x.device)
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    run = fmm.evaluate_field_gradient_adjoint_device if adjoint else fmm.evaluate_field_gradient_device
+    for k0 in range(0, x.shape[0], _MAX_BATCH):
+        k1 = min(k0 + _MAX_BATCH, x.shape[0])
+        run(x[k0:k1].data_ptr(), out[k0:k1].data_ptr(), k1 - k0, stream)
+    return out
+
+
+def _contract(grad6, v):
+    """G v for G stored as (..., 6) xx xy xz yy yz zz and v (..., 3)."""
+    return torch.stack([sum(grad6[..., j] * v[..., i] for i, j in enumerate(row)) for row in _GRAD_PAIRS], dim=-1)
+
+
+def field_gradient(fmm, moments):
+    """Field gradient dH_c/dx_j at the targets: (K, n_sources, 3) device
+    moments -> (K, n_targets, 6) as xx xy xz yy yz zz, in the plan precision."""
+    if torch is None:
+        raise RuntimeError("field_gradient needs torch")
+    if not moments.is_cuda or moments.dim() != 3 or moments.shape[1:] != (fmm.source_count, 3):
+        raise ValueError("moments must be a (batch, n_sources, 3) device tensor")
+    return _apply_gradient_device(fmm, moments, fmm.target_count, adjoint=False)
+
+
+def position_gradients(fmm, moments, cotangents):
+    """dL/d(target positions) (n_targets, 3) and dL/d(source positions)
+    (n_sources, 3) of L = sum over states and targets of cotangents . fields.
+
+    moments: (K, n_sources, 3) and cotangents: (K, n_targets, 3) device
+    tensors in user order.  Where a target and a source are one point (the
+    self-identity map of the plan), its position gradient is the sum of both.
+    """
+    if torch is None:
+        raise RuntimeError("position_gradients needs torch")
+    ns, nt = fmm.source_count, fmm.target_count
+    if not moments.is_cuda or moments.dim() != 3 or moments.shape[1:] != (ns, 3):
+        raise ValueError("moments must be a (batch, n_sources, 3) device tensor")
+    if not cotangents.is_cuda or cotangents.shape != (moments.shape[0], nt, 3):
+        raise ValueError("cotangents must be a (batch, n_targets, 3) device tensor")
+    dt = _plan_dtype(fmm)
+    g_t = _apply_gradient_device(fmm, moments, nt, adjoint=False)
+    g_s = _apply_gradient_device(fmm, cotangents, ns, adjoint=True)
+    d_targets = _contract(g_t, cotangents.to(dt)).sum(dim=0)
+    d_sources = _contract(g_s, moments.to(dt)).sum(dim=0)
+    return d_targets, d_sources

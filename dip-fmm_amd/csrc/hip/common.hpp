@@ -153,6 +153,9 @@ struct DevicePlan {
   void* Lc = nullptr;
   void* Hn = nullptr;
   void* Pn = nullptr;          // near-field potential Pn[target][K] (allocated once a potential is asked for)
+  void* Gn = nullptr;          // near-field field gradient Gn[target][6][K] (allocated once asked for)
+  void* gradients_out = nullptr;  // host-path staging: K x nt x 6
+  int* grad_tab = nullptr;     // [n][6]: index of b - e_c - e_j (xx xy xz yy yz zz), or -1
   double potential_scale = 1.0;  // frame factor of the potential (root width W for FP32 plans)
   // Harmonic-subspace M2L: L = Q sum_o C_o P M with the level scalings folded
   // into P and Q (nc = 0: the M2L runs on the full n coefficients).
@@ -206,6 +209,12 @@ template <class S>
 void launch_m2l_valu(const DevicePlan& d, int K, hipStream_t s);  // all levels 2..L
 template <class S>
 void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s);
+// Field gradient dH_c/dx_j: near part into Gn, far part from the leaf locals
+// plus Gn, unsorted to out[k][user target][6] (xx xy xz yy yz zz).
+template <class S>
+void launch_p2p_grad(const DevicePlan& d, int K, hipStream_t s);
+template <class S>
+void launch_l2p_grad_unsort(const DevicePlan& d, S* gradients, int K, hipStream_t s);
 // Far field from the leaf locals plus the near field, unsorted to user
 // order: fields out[k][target][3] (may be null when potentials are asked
 // for) and potentials pot[k][target] (null: fields only).

@@ -556,6 +556,70 @@ inline unsigned tile_threads(int row_tiles, int KP) {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Far-field gradient of the field from the leaf local (no counterpart in
+// the reference): the derivative of its L2P polynomial,
+//   G_cj(t) = -sum_b L[b] d_t^(b - e_c - e_j) / (b - e_c - e_j)!,
+// entries xx xy xz yy yz zz (grad_tab[b][e]: index of b - e_c - e_j, or -1),
+// plus the near-field gradient Gn, times the frame factor, unsorted to
+// out[k][user target][6].  One workgroup per target leaf, targets in
+// chunks of 32; thread per (target, state).
+template <class S>
+__global__ void __launch_bounds__(256) l2p_grad_kernel(const DevicePlan d, S* __restrict__ out, int K, int nm2,
+                                                       S scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = d.n, PW = max(d.p - 1, 1);
+  int* gt = reinterpret_cast<int*>(smem_raw);                      // [n][6]
+  S* pw = reinterpret_cast<S*>(smem_raw + align16(size_t(n) * 6 * sizeof(int)));  // [32][3][PW]
+  S* mono = pw + 32 * 3 * PW;                                       // [32][nm2]
+  const int leaf = blockIdx.x, tid = threadIdx.x;
+  const int2 range = d.leaf_tgt[leaf];
+  if (range.x == range.y) return;
+  const int64_t node = level_offset(d.L) + leaf;
+  for (int i = tid; i < n * 6; i += 256) gt[i] = d.grad_tab[i];
+  const auto* rel = static_cast<const typename Vec4<S>::type*>(d.tgt_rel);
+  const S* Lb = static_cast<const S*>(d.Lc) + node * int64_t(n) * K;
+  const S* Gn = static_cast<const S*>(d.Gn);
+  for (int t0 = range.x; t0 < range.y; t0 += 32) {
+    const int cnt = min(32, range.y - t0);
+    __syncthreads();
+    if (tid < 96) {
+      const int t = tid / 3, a = tid % 3;
+      S x = S(0);
+      if (t < cnt) {
+        const auto r = rel[t0 + t];
+        x = a == 0 ? r.x : (a == 1 ? r.y : r.z);
+      }
+      for (int i = 0; i < PW; ++i) pw[tid * PW + i] = axis_power<S>(x, i);
+    }
+    __syncthreads();
+    for (int w = tid; w < 32 * nm2; w += 256) {
+      const int t = w / nm2, g = w - t * nm2;
+      const int4 e = d.p2m_tab[n + g];
+      const S* pt = pw + t * 3 * PW;
+      mono[w] = pt[e.x] * pt[PW + e.y] * pt[2 * PW + e.z];
+    }
+    __syncthreads();
+    for (int item = tid; item < cnt * K; item += 256) {
+      const int t = item / K, k = item - t * K;
+      S G[6] = {S(0), S(0), S(0), S(0), S(0), S(0)};
+      const S* mt = mono + t * nm2;
+      for (int b = 0; b < n; ++b) {
+        const S lv = Lb[int64_t(b) * K + k];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+          const int i = gt[b * 6 + e];
+          if (i >= 0) G[e] -= lv * mt[i];
+        }
+      }
+      const int ts = t0 + t;
+      const int user = d.tgt_perm[ts];
+      S* o = out + (int64_t(k) * d.nt + user) * 6;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) o[e] = scale * (Gn[(int64_t(ts) * 6 + e) * K + k] + G[e]);
+    }
+  }
+}
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // P2M on the matrix cores (FP32 plans, K a multiple of 32).  With
@@ -855,6 +919,17 @@ void launch_p2m(const DevicePlan& d, int K, hipStream_t s) {
 }
 
 template <class S>
+void launch_l2p_grad_unsort(const DevicePlan& d, S* gradients, int K, hipStream_t s) {
+  if (d.leaves == 0 || d.nt == 0) return;
+  if (d.Gn == nullptr || d.grad_tab == nullptr) throw invalid_argument("field-gradient buffers are not allocated");
+  const int nm2 = d.p >= 2 ? coefficient_count(d.p - 2) : 0;
+  const size_t lds = align16(size_t(d.n) * 6 * sizeof(int)) + (size_t(32) * 3 * std::max(d.p - 1, 1) + size_t(32) * std::max(nm2, 1)) * sizeof(S);
+  const S scale = S(1.0 / d.potential_scale);
+  l2p_grad_kernel<S><<<unsigned(d.leaves), 256, lds, s>>>(d, gradients, K, nm2, scale);
+  HIPFMM_CHECK(hipGetLastError());
+}
+
+template <class S>
 void launch_m2m(const DevicePlan& d, int parent_level, int K, hipStream_t s) {
   const int NP = round_up(d.n, kRA);
   const int kc = state_chunk(K, [&](int c) { return size_t(d.n) * round_up(c, kRK) * sizeof(S); });
@@ -953,6 +1028,7 @@ void launch_zero(S* ptr, size_t count, hipStream_t s) {
   template void launch_m2m<S>(const DevicePlan&, int, int, hipStream_t);           \
   template void launch_l2l<S>(const DevicePlan&, int, int, hipStream_t);           \
   template void launch_l2p_unsort<S>(const DevicePlan&, S*, S*, int, hipStream_t);     \
+  template void launch_l2p_grad_unsort<S>(const DevicePlan&, S*, int, hipStream_t);    \
   template void launch_m2l_compress<S>(const DevicePlan&, int, bool, int, hipStream_t); \
   template void launch_m2l_expand<S>(const DevicePlan&, int, int, hipStream_t);    \
   template void launch_zero<S>(S*, size_t, hipStream_t);

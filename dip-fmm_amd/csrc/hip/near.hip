@@ -535,6 +535,132 @@ void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s) {
 template void launch_p2p<float>(const DevicePlan&, int, bool, hipStream_t);
 template void launch_p2p<double>(const DevicePlan&, int, bool, hipStream_t);
 
+namespace {
+
+// Near-field gradient of the field, G_cj = dH_c/dx_j of every target over
+// its list-1 sources (no counterpart in the reference; the derivative of its
+// pair tensor, src/cuboid.cpp:293-307): with r = x_t - x_s, u = r/|r|,
+// w = u.m and g = 3/(4 pi |r|^4),
+//   G_cj += g (delta_cj w + u_c m_j + m_c u_j - 5 u_c u_j w),
+// symmetric and traceless; entries xx xy xz yy yz zz.  Identity pair skipped,
+// coincident pairs NaN.  Workgroup per target leaf as the field kernel:
+// thread (slot, state group), sources staged in LDS, into Gn[t][6][K].
+template <class S, int KPT>
+__global__ void __launch_bounds__(kThreads) p2p_grad_kernel(const DevicePlan d, int K, int KG, int chunk) {
+  extern __shared__ unsigned char smem_raw[];
+  auto* spos = reinterpret_cast<PairPos<S>*>(smem_raw);  // chunk
+  S* smom = reinterpret_cast<S*>(spos + chunk);           // chunk x 3 x K
+  const int leaf = blockIdx.x;
+  const int2 trange = d.leaf_tgt[leaf];
+  if (trange.x == trange.y) return;
+  const int slot = threadIdx.x % kSlots, g = threadIdx.x / kSlots;
+  const int kg = g % KG, sp = g / KG, SP = 8 / KG;
+  const int nb = 1 << d.L;
+  int lx, ly, lz;
+  demorton3(uint32_t(leaf), d.L, lx, ly, lz);
+  const S c34pi = S(0.23873241463784300365332564505877154);  // 3/(4 pi)
+  const S* sm = static_cast<const S*>(d.sm);
+  S* Gn = static_cast<S*>(d.Gn);
+  S* red = smom;
+  for (int t0 = trange.x; t0 < trange.y; t0 += kSlots) {
+    const int t = t0 + slot;
+    const bool active = t < trange.y;
+    const PairPos<S> tp = PairPos<S>::load(d.tgt_pos, d.nt, active ? t : trange.x);
+    const int self = active ? d.self[t] : -2;
+    S acc[KPT][6];
+#pragma unroll
+    for (int q = 0; q < KPT; ++q)
+#pragma unroll
+      for (int e = 0; e < 6; ++e) acc[q][e] = S(0);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int x = lx + dx, y = ly + dy, z = lz + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+          const int2 sr = d.leaf_src[morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L)];
+          for (int j0 = sr.x; j0 < sr.y; j0 += chunk) {
+            const int cnt = min(chunk, sr.y - j0);
+            __syncthreads();
+            for (int w = threadIdx.x; w < cnt; w += kThreads) spos[w] = PairPos<S>::load(d.src_pos, d.ns, j0 + w);
+            for (int w = threadIdx.x; w < cnt * 3 * K; w += kThreads) smom[w] = sm[int64_t(j0) * 3 * K + w];
+            __syncthreads();
+            if (active) {
+              for (int j = sp; j < cnt; j += SP) {
+                if (j0 + j == self) continue;
+                S rx, ry, rz;
+                tp.diff(spos[j], rx, ry, rz);
+                const S r2 = rx * rx + ry * ry + rz * rz;
+                const S rinv = inv_sqrt(r2);
+                const S gg = c34pi * rinv * rinv * rinv * rinv;
+                const S ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
+                const S* mj = smom + j * 3 * K + kg * KPT;
+#pragma unroll
+                for (int q = 0; q < KPT; ++q) {
+                  if (kg * KPT + q < K) {
+                    const S mx = mj[q], my = mj[K + q], mz = mj[2 * K + q];
+                    const S w = ux * mx + uy * my + uz * mz;
+                    const S w5 = S(5) * w;
+                    acc[q][0] += gg * (w + S(2) * ux * mx - w5 * ux * ux);
+                    acc[q][1] += gg * (ux * my + mx * uy - w5 * ux * uy);
+                    acc[q][2] += gg * (ux * mz + mx * uz - w5 * ux * uz);
+                    acc[q][3] += gg * (w + S(2) * uy * my - w5 * uy * uy);
+                    acc[q][4] += gg * (uy * mz + my * uz - w5 * uy * uz);
+                    acc[q][5] += gg * (w + S(2) * uz * mz - w5 * uz * uz);
+                  }
+                }
+              }
+            }
+          }
+        }
+    __syncthreads();
+    if (SP > 1) {
+      for (int q = 0; q < KPT; ++q)
+        for (int e = 0; e < 6; ++e) red[((g * kSlots + slot) * KPT + q) * 6 + e] = acc[q][e];
+      __syncthreads();
+      if (sp == 0 && active)
+        for (int s2 = 1; s2 < SP; ++s2) {
+          const int g2 = s2 * KG + kg;
+          for (int q = 0; q < KPT; ++q)
+            for (int e = 0; e < 6; ++e) acc[q][e] += red[((g2 * kSlots + slot) * KPT + q) * 6 + e];
+        }
+    }
+    if (sp == 0 && active)
+      for (int q = 0; q < KPT; ++q) {
+        const int k = kg * KPT + q;
+        if (k < K)
+          for (int e = 0; e < 6; ++e) Gn[(int64_t(t) * 6 + e) * K + k] = acc[q][e];
+      }
+  }
+}
+
+template <class S, int KPT>
+void launch_p2p_grad_t(const DevicePlan& d, int K, hipStream_t s) {
+  int KG = (K + KPT - 1) / KPT;
+  int kg = 1;
+  while (kg < KG) kg *= 2;
+  const int chunk = sizeof(S) == 4 ? 64 : 32;
+  const size_t stage = size_t(chunk) * sizeof(PairPos<S>) + size_t(chunk) * 3 * K * sizeof(S);
+  const size_t reduce = size_t(8) * kSlots * KPT * 6 * sizeof(S) + size_t(chunk) * sizeof(PairPos<S>);
+  p2p_grad_kernel<S, KPT><<<unsigned(d.leaves), kThreads, std::max(stage, reduce), s>>>(d, K, kg, chunk);
+  HIPFMM_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+template <class S>
+void launch_p2p_grad(const DevicePlan& d, int K, hipStream_t s) {
+  if (d.leaves == 0 || d.nt == 0) return;
+  if (d.Gn == nullptr) throw invalid_argument("near-field gradient buffer is not allocated");
+  if (K <= 8)
+    launch_p2p_grad_t<S, 1>(d, K, s);
+  else if (K <= 32)
+    launch_p2p_grad_t<S, 4>(d, K, s);
+  else
+    launch_p2p_grad_t<S, 8>(d, K, s);
+}
+template void launch_p2p_grad<float>(const DevicePlan&, int, hipStream_t);
+template void launch_p2p_grad<double>(const DevicePlan&, int, hipStream_t);
+
 void direct_field_fp64(const double* targets, int nt, const double* sources, int ns, const double* moments,
                        const int* self, double* fields) {
   if (nt == 0) return;

@@ -70,6 +70,9 @@ struct hipfmm_plan {
   int* adj_self = nullptr;
   bool adj_self_valid = true;  // false when two targets share one source
   bool potential_used = false;  // keep the near-field potential buffers in the state
+  bool gradient_used = false;   // keep the field-gradient buffers in the state
+  void* pinned_grad = nullptr;
+  size_t pinned_grad_bytes = 0;
   void* pinned_pot = nullptr;
   size_t pinned_pot_bytes = 0;
   double device_moment_scale = 1.0;
@@ -363,6 +366,28 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   d.self = upload(P, self.data(), self.size());
   const auto tab = basis_tables(p);
   d.p2m_tab = upload(P, tab.data(), tab.size());
+  // Field-gradient table: b -> b - e_c - e_j for (c, j) = xx xy xz yy yz zz.
+  {
+    std::vector<int> gt(size_t(n) * 6, -1);
+    const int cj[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int b = 0; b < n; ++b) {
+      const int4 e = tab[n + b];
+      for (int q = 0; q < 6; ++q) {
+        int x[3] = {e.x, e.y, e.z};
+        --x[cj[q][0]];
+        --x[cj[q][1]];
+        if (x[0] < 0 || x[1] < 0 || x[2] < 0) continue;
+        for (int g = 0; g < n; ++g) {
+          const int4 f = tab[n + g];
+          if (f.x == x[0] && f.y == x[1] && f.z == x[2]) {
+            gt[size_t(b) * 6 + q] = g;
+            break;
+          }
+        }
+      }
+    }
+    d.grad_tab = upload(P, gt.data(), gt.size());
+  }
   // Raising table of the matrix-core L2P: g (degree < p) -> g + e_c.
   {
     std::vector<int4> up(std::max(d.nm, 1), int4{0, 0, 0, 0});
@@ -616,13 +641,14 @@ void free_state(hipfmm_plan* P) {
 }
 
 template <class S>
-void ensure_capacity(hipfmm_plan* P, int K, bool potential) {
+void ensure_capacity(hipfmm_plan* P, int K, bool potential, bool gradient = false) {
   DevicePlan& d = P->d;
-  if (potential && !P->potential_used) {
-    P->potential_used = true;
+  if ((potential && !P->potential_used) || (gradient && !P->gradient_used)) {
+    P->potential_used = P->potential_used || potential;
+    P->gradient_used = P->gradient_used || gradient;
     HIPFMM_CHECK(hipStreamSynchronize(P->s_far));
     HIPFMM_CHECK(hipStreamSynchronize(P->s_near));
-    free_state(P);  // reallocated below with the potential buffers
+    free_state(P);  // reallocated below with the extra buffers
   }
   if (K <= d.kcap) return;
   // Earlier evaluates may still be running on the plan's streams.
@@ -646,6 +672,12 @@ void ensure_capacity(hipfmm_plan* P, int K, bool potential) {
   if (P->potential_used) {
     d.Pn = alloc(points * cap, sizeof(S));
     d.potentials_out = alloc(points * cap, sizeof(S));
+  }
+  d.Gn = nullptr;
+  d.gradients_out = nullptr;
+  if (P->gradient_used) {
+    d.Gn = alloc(points * 6 * cap, sizeof(S));
+    d.gradients_out = alloc(points * 6 * cap, sizeof(S));
   }
   d.M = alloc(cs, sizeof(S));
   d.Lc = alloc(cs, sizeof(S));
@@ -733,15 +765,19 @@ DevicePlan role_exchanged(const hipfmm_plan* P) {
 // and nothing waits on the host.
 template <class S>
 void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* potentials, int K, const int* self,
-                   int memspace, hipStream_t user, bool adjoint, bool capture = false) {
+                   int memspace, hipStream_t user, bool adjoint, bool capture = false,
+                   void* field_gradients = nullptr) {
   const bool want_pot = potentials != nullptr;
+  const bool want_grad = field_gradients != nullptr;
   {
     const DevicePlan& f = P->d;
     const int n_in = adjoint ? f.nt : f.ns, n_out = adjoint ? f.ns : f.nt;
     require(K >= 1 && K <= hipfmm::kMaxBatch, "batch must be between 1 and 64 moment states");
     require(moments != nullptr || n_in == 0, adjoint ? "cotangents pointer is null" : "moments pointer is null");
     require(!adjoint || potentials == nullptr, "the adjoint is of the field map only");
-    require(want_pot || fields != nullptr || n_out == 0,
+    require(!want_grad || (fields == nullptr && potentials == nullptr),
+            "the field gradient is evaluated on its own");
+    require(want_pot || want_grad || fields != nullptr || n_out == 0,
             adjoint ? "gradients pointer is null" : "fields pointer is null");
     require(memspace == HIPFMM_HOST || memspace == HIPFMM_DEVICE, "unknown memory space");
     // Identity map: fixed with the plan on first use (src/cuda_fmm.cu:3470-3486).
@@ -766,9 +802,10 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     // No wait on earlier evaluates here: their stage events are read back
     // only when timings are asked for, so the host can enqueue ahead.
     (void)hipGetLastError();
-    ensure_capacity<S>(P, K, want_pot);
+    ensure_capacity<S>(P, K, want_pot, want_grad);
   } else {
-    require(K <= P->d.kcap && (!want_pot || P->potential_used), "state is not allocated for a captured evaluate");
+    require(K <= P->d.kcap && (!want_pot || P->potential_used) && (!want_grad || P->gradient_used),
+            "state is not allocated for a captured evaluate");
   }
   const DevicePlan d = adjoint ? role_exchanged(P) : P->d;
   hipStream_t s = P->s_far;
@@ -779,12 +816,20 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   const S* mu = nullptr;
   S* out = nullptr;
   S* pot = nullptr;
+  S* grad = nullptr;
   S scale = S(1);
   const size_t in_bytes = size_t(K) * d.ns * 3 * sizeof(S);
   const size_t out_bytes = fields != nullptr ? size_t(K) * d.nt * 3 * sizeof(S) : 0;
   const size_t pot_bytes = want_pot ? size_t(K) * d.nt * sizeof(S) : 0;
+  const size_t grad_bytes = want_grad ? size_t(K) * d.nt * 6 * sizeof(S) : 0;
   const bool host = memspace == HIPFMM_HOST;
   if (host) {
+    if (P->pinned_grad_bytes < grad_bytes) {
+      if (P->pinned_grad) (void)hipHostFree(P->pinned_grad);
+      P->pinned_grad = nullptr;
+      HIPFMM_CHECK(hipHostMalloc(&P->pinned_grad, std::max<size_t>(grad_bytes, 16)));
+      P->pinned_grad_bytes = grad_bytes;
+    }
     if (P->pinned_pot_bytes < pot_bytes) {
       if (P->pinned_pot) (void)hipHostFree(P->pinned_pot);
       P->pinned_pot = nullptr;
@@ -810,8 +855,9 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     mu = static_cast<const S*>(d.moments_in);
     out = fields != nullptr ? static_cast<S*>(d.fields_out) : nullptr;
     pot = want_pot ? static_cast<S*>(d.potentials_out) : nullptr;
+    grad = want_grad ? static_cast<S*>(d.gradients_out) : nullptr;
     P->stats.evaluation_h2d_bytes = in_bytes;
-    P->stats.evaluation_d2h_bytes = out_bytes + pot_bytes;
+    P->stats.evaluation_d2h_bytes = out_bytes + pot_bytes + grad_bytes;
     P->stats.evaluation_h2d_calls += 1;
     P->stats.evaluation_d2h_calls += 1;
   } else {
@@ -822,6 +868,7 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     mu = static_cast<const S*>(moments);
     out = static_cast<S*>(fields);
     pot = static_cast<S*>(potentials);
+    grad = static_cast<S*>(field_gradients);
     scale = S(P->device_moment_scale);
     P->stats.evaluation_h2d_bytes = 0;
     P->stats.evaluation_d2h_bytes = 0;
@@ -830,7 +877,10 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   HIPFMM_CHECK(hipEventRecord(e[kEvGather], s));
   HIPFMM_CHECK(hipStreamWaitEvent(P->s_near, e[kEvGather], 0));
   mark(kEvNear0, P->s_near);
-  hipfmm::launch_p2p<S>(d, K, want_pot, P->s_near);
+  if (want_grad)
+    hipfmm::launch_p2p_grad<S>(d, K, P->s_near);
+  else
+    hipfmm::launch_p2p<S>(d, K, want_pot, P->s_near);
   HIPFMM_CHECK(hipEventRecord(e[kEvNear1], P->s_near));
   hipfmm::launch_p2m<S>(d, K, s);
   mark(kEvP2M, s);
@@ -905,7 +955,10 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   }
   mark(kEvL2L, s);
   HIPFMM_CHECK(hipStreamWaitEvent(s, e[kEvNear1], 0));
-  hipfmm::launch_l2p_unsort<S>(d, out, pot, K, s);
+  if (want_grad)
+    hipfmm::launch_l2p_grad_unsort<S>(d, grad, K, s);
+  else
+    hipfmm::launch_l2p_unsort<S>(d, out, pot, K, s);
   HIPFMM_CHECK(hipEventRecord(e[kEvL2P], s));
   P->last_K = K;
   P->last_host = host;
@@ -914,10 +967,13 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     HIPFMM_CHECK(hipEventRecord(e[kEvD2H0], s));
     if (out_bytes) HIPFMM_CHECK(hipMemcpyAsync(P->pinned_out, d.fields_out, out_bytes, hipMemcpyDeviceToHost, s));
     if (pot_bytes) HIPFMM_CHECK(hipMemcpyAsync(P->pinned_pot, d.potentials_out, pot_bytes, hipMemcpyDeviceToHost, s));
+    if (grad_bytes)
+      HIPFMM_CHECK(hipMemcpyAsync(P->pinned_grad, d.gradients_out, grad_bytes, hipMemcpyDeviceToHost, s));
     HIPFMM_CHECK(hipEventRecord(e[kEvD2H1], s));
     HIPFMM_CHECK(hipStreamSynchronize(s));
     if (out_bytes) std::memcpy(fields, P->pinned_out, out_bytes);
     if (pot_bytes) std::memcpy(potentials, P->pinned_pot, pot_bytes);
+    if (grad_bytes) std::memcpy(field_gradients, P->pinned_grad, grad_bytes);
     finalize_timings(P);
   } else {
     HIPFMM_CHECK(hipStreamWaitEvent(user, e[kEvL2P], 0));
@@ -1035,6 +1091,38 @@ int hipfmm_plan_evaluate_potential(hipfmm_plan* plan, const void* moments, void*
       evaluate_impl<float>(plan, moments, fields, potentials, batch, sorted_self_indices, memspace, user, false);
     else
       evaluate_impl<double>(plan, moments, fields, potentials, batch, sorted_self_indices, memspace, user, false);
+  });
+}
+
+int hipfmm_plan_evaluate_field_gradient(hipfmm_plan* plan, const void* moments, void* gradients, int batch,
+                                        const int* sorted_self_indices, int memspace, void* stream) {
+  return guarded([&] {
+    require(plan != nullptr, "plan is null");
+    require(gradients != nullptr || plan->d.nt == 0, "gradients pointer is null");
+    if (gradients == nullptr) return;
+    DeviceGuard guard(plan->device);
+    const hipStream_t user = static_cast<hipStream_t>(stream);
+    if (plan->d.precision == 32)
+      evaluate_impl<float>(plan, moments, nullptr, nullptr, batch, sorted_self_indices, memspace, user, false, false,
+                           gradients);
+    else
+      evaluate_impl<double>(plan, moments, nullptr, nullptr, batch, sorted_self_indices, memspace, user, false, false,
+                            gradients);
+  });
+}
+
+int hipfmm_plan_evaluate_field_gradient_adjoint(hipfmm_plan* plan, const void* cotangents, void* gradients,
+                                                int batch, int memspace, void* stream) {
+  return guarded([&] {
+    require(plan != nullptr, "plan is null");
+    require(gradients != nullptr || plan->d.ns == 0, "gradients pointer is null");
+    if (gradients == nullptr) return;
+    DeviceGuard guard(plan->device);
+    const hipStream_t user = static_cast<hipStream_t>(stream);
+    if (plan->d.precision == 32)
+      evaluate_impl<float>(plan, cotangents, nullptr, nullptr, batch, nullptr, memspace, user, true, false, gradients);
+    else
+      evaluate_impl<double>(plan, cotangents, nullptr, nullptr, batch, nullptr, memspace, user, true, false, gradients);
   });
 }
 
@@ -1192,6 +1280,7 @@ void hipfmm_plan_destroy(hipfmm_plan* plan) {
   if (plan->pinned_in) (void)hipHostFree(plan->pinned_in);
   if (plan->pinned_out) (void)hipHostFree(plan->pinned_out);
   if (plan->pinned_pot) (void)hipHostFree(plan->pinned_pot);
+  if (plan->pinned_grad) (void)hipHostFree(plan->pinned_grad);
   for (auto& ev : plan->ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto& pair : plan->m2l_ring)

@@ -622,6 +622,71 @@ void UniformFmm::evaluate_adjoint_device(const void* cotangents, void* gradients
   check(hipfmm_plan_evaluate_adjoint(plan_, cotangents, gradients, batch, HIPFMM_DEVICE, stream));
 }
 
+namespace {
+
+// Vectors in the plan precision, each divided by `divisor` (the FP32 frame
+// scales moments and cotangents by the cube of its length unit).
+template <typename S>
+std::vector<S> flatten(std::span<const Vec3> v, double divisor) {
+  std::vector<S> out(3 * v.size());
+  for (std::size_t i = 0; i < v.size(); ++i) {
+    out[3 * i] = static_cast<S>(v[i].x / divisor);
+    out[3 * i + 1] = static_cast<S>(v[i].y / divisor);
+    out[3 * i + 2] = static_cast<S>(v[i].z / divisor);
+  }
+  return out;
+}
+
+}  // namespace
+
+void UniformFmm::evaluate_field_gradient_batch_into(std::span<const Vec3> moments, int batch, void* gradients,
+                                                    std::span<const int> ids) {
+  const std::size_t ns = tree_.sorted_sources().size();
+  if (batch < 1 || moments.size() != ns * static_cast<std::size_t>(batch))
+    throw std::invalid_argument("batched moments must hold batch x source_count vectors");
+  ids = resolve_self_indices(ids);
+  validate_identities(ids, ns);
+  const auto self = sorted_self_indices(ids);
+  if (precision_ == StaticPrecision::Float32) {
+    const auto m = flatten<float>(moments, scale_ * scale_ * scale_);
+    check(hipfmm_plan_evaluate_field_gradient(plan_, m.data(), gradients, batch, self.data(), HIPFMM_HOST, nullptr));
+  } else {
+    const auto m = flatten<double>(moments, 1.0);
+    check(hipfmm_plan_evaluate_field_gradient(plan_, m.data(), gradients, batch, self.data(), HIPFMM_HOST, nullptr));
+  }
+  record_timings();
+}
+
+void UniformFmm::evaluate_field_gradient_device(const void* moments, void* gradients, int batch, void* stream) {
+  const int* self = nullptr;
+  std::vector<int> s;
+  if (fixed_ids_.has_value()) {
+    s = sorted_self_indices(fixed_ids_.value());
+    self = s.data();
+  }
+  check(hipfmm_plan_evaluate_field_gradient(plan_, moments, gradients, batch, self, HIPFMM_DEVICE, stream));
+}
+
+void UniformFmm::evaluate_field_gradient_adjoint_batch_into(std::span<const Vec3> cotangents, int batch,
+                                                            void* gradients) {
+  const std::size_t nt = tree_.sorted_targets().size();
+  if (batch < 1 || cotangents.size() != nt * static_cast<std::size_t>(batch))
+    throw std::invalid_argument("batched cotangents must hold batch x target_count vectors");
+  if (precision_ == StaticPrecision::Float32) {
+    const auto g = flatten<float>(cotangents, scale_ * scale_ * scale_);
+    check(hipfmm_plan_evaluate_field_gradient_adjoint(plan_, g.data(), gradients, batch, HIPFMM_HOST, nullptr));
+  } else {
+    const auto g = flatten<double>(cotangents, 1.0);
+    check(hipfmm_plan_evaluate_field_gradient_adjoint(plan_, g.data(), gradients, batch, HIPFMM_HOST, nullptr));
+  }
+  record_timings();
+}
+
+void UniformFmm::evaluate_field_gradient_adjoint_device(const void* cotangents, void* gradients, int batch,
+                                                        void* stream) {
+  check(hipfmm_plan_evaluate_field_gradient_adjoint(plan_, cotangents, gradients, batch, HIPFMM_DEVICE, stream));
+}
+
 void UniformFmm::set_m2l_mode(HipM2LMode mode) {
   check(hipfmm_plan_set_m2l_mode(plan_, static_cast<int>(mode)));
   m2l_mode_ = mode;

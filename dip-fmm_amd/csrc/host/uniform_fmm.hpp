@@ -197,6 +197,16 @@ class UniformFmm {
   void evaluate_adjoint_batch_into(std::span<const Vec3> cotangents, int batch, void* gradients);
   // Device-resident adjoint (K x N_t x 3 in, K x N_s x 3 out).
   void evaluate_adjoint_device(const void* cotangents, void* gradients, int batch, void* stream);
+  // Field gradient G_cj = dH_c/dx_j at the targets, K x N_t x 6 scalars of
+  // the plan precision (xx xy xz yy yz zz), for gradients with respect to
+  // positions (no counterpart in the reference).
+  void evaluate_field_gradient_batch_into(std::span<const Vec3> moments, int batch, void* gradients,
+                                          std::span<const int> target_source_indices = {});
+  void evaluate_field_gradient_device(const void* moments, void* gradients, int batch, void* stream);
+  // The field gradient at the sources of the role-exchanged problem whose
+  // dipoles are the cotangents (K x N_t vectors in, K x N_s x 6 out).
+  void evaluate_field_gradient_adjoint_batch_into(std::span<const Vec3> cotangents, int batch, void* gradients);
+  void evaluate_field_gradient_adjoint_device(const void* cotangents, void* gradients, int batch, void* stream);
   // Stage times of the most recent evaluate (waits for a device-resident
   // evaluate to finish).
   EvaluationTimings device_timings();

@@ -428,6 +428,83 @@ m.def("hip_full_available", &hip_full_available);
           py::arg("cotangents_ptr"), py::arg("gradients_ptr"), py::arg("batch"), py::arg("stream") = 0,
           "Device-resident adjoint: pointers to (batch, n_targets, 3) cotangents and (batch, n_sources, 3) "
           "gradients of the plan precision; ordered after `stream`.")
+      .def(
+          "evaluate_field_gradient_batch",
+          [](UniformFmm& f, py::object moments, py::object ids) -> py::object {
+            const DoubleArray a = py::cast<DoubleArray>(moments);
+            const auto info = a.request();
+            if (info.ndim != 3 || info.shape[2] != 3)
+              throw std::invalid_argument("moments must have shape (batch, n_sources, 3)");
+            const int K = static_cast<int>(info.shape[0]);
+            std::vector<Vec3> m(static_cast<std::size_t>(K * info.shape[1]));
+            const double* p = a.data();
+            for (std::size_t i = 0; i < m.size(); ++i) m[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+            std::vector<int> self;
+            if (!ids.is_none()) self = py::cast<std::vector<int>>(ids);
+            const py::ssize_t nt = static_cast<py::ssize_t>(f.tree().sorted_targets().size());
+            auto run = [&](auto* o) {
+              py::gil_scoped_release release;
+              f.evaluate_field_gradient_batch_into(m, K, o, self);
+            };
+            if (f.precision() == StaticPrecision::Float32) {
+              py::array_t<float> out({py::ssize_t(K), nt, py::ssize_t(6)});
+              run(out.mutable_data());
+              return out;
+            }
+            py::array_t<double> out({py::ssize_t(K), nt, py::ssize_t(6)});
+            run(out.mutable_data());
+            return out;
+          },
+          py::arg("moments"), py::arg("target_source_indices") = py::none(),
+          "Field gradient dH_c/dx_j at the targets: (batch, n_targets, 6) as xx xy xz yy yz zz.")
+      .def(
+          "evaluate_field_gradient_adjoint_batch",
+          [](UniformFmm& f, py::object cotangents) -> py::object {
+            const DoubleArray a = py::cast<DoubleArray>(cotangents);
+            const auto info = a.request();
+            if (info.ndim != 3 || info.shape[2] != 3)
+              throw std::invalid_argument("cotangents must have shape (batch, n_targets, 3)");
+            const int K = static_cast<int>(info.shape[0]);
+            std::vector<Vec3> g(static_cast<std::size_t>(K * info.shape[1]));
+            const double* p = a.data();
+            for (std::size_t i = 0; i < g.size(); ++i) g[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+            const py::ssize_t ns = static_cast<py::ssize_t>(f.tree().sorted_sources().size());
+            auto run = [&](auto* o) {
+              py::gil_scoped_release release;
+              f.evaluate_field_gradient_adjoint_batch_into(g, K, o);
+            };
+            if (f.precision() == StaticPrecision::Float32) {
+              py::array_t<float> out({py::ssize_t(K), ns, py::ssize_t(6)});
+              run(out.mutable_data());
+              return out;
+            }
+            py::array_t<double> out({py::ssize_t(K), ns, py::ssize_t(6)});
+            run(out.mutable_data());
+            return out;
+          },
+          py::arg("cotangents"),
+          "Field gradient at the sources of the role-exchanged problem whose dipoles are the cotangents: "
+          "(batch, n_sources, 6).")
+      .def(
+          "evaluate_field_gradient_device",
+          [](UniformFmm& f, std::uintptr_t moments, std::uintptr_t gradients, int batch, std::uintptr_t stream) {
+            py::gil_scoped_release release;
+            f.evaluate_field_gradient_device(reinterpret_cast<const void*>(moments),
+                                             reinterpret_cast<void*>(gradients), batch,
+                                             reinterpret_cast<void*>(stream));
+          },
+          py::arg("moments_ptr"), py::arg("gradients_ptr"), py::arg("batch"), py::arg("stream") = 0,
+          "Device-resident field gradient: (batch, n_sources, 3) moments -> (batch, n_targets, 6).")
+      .def(
+          "evaluate_field_gradient_adjoint_device",
+          [](UniformFmm& f, std::uintptr_t cotangents, std::uintptr_t gradients, int batch, std::uintptr_t stream) {
+            py::gil_scoped_release release;
+            f.evaluate_field_gradient_adjoint_device(reinterpret_cast<const void*>(cotangents),
+                                                     reinterpret_cast<void*>(gradients), batch,
+                                                     reinterpret_cast<void*>(stream));
+          },
+          py::arg("cotangents_ptr"), py::arg("gradients_ptr"), py::arg("batch"), py::arg("stream") = 0,
+          "Device-resident adjoint field gradient: (batch, n_targets, 3) cotangents -> (batch, n_sources, 6).")
       .def("device_timings", [](UniformFmm& f) {
         EvaluationTimings t;
         {

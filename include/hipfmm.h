@@ -239,6 +239,23 @@ int hipfmm_plan_evaluate_potential(hipfmm_plan* plan, const void* moments, void*
 int hipfmm_plan_evaluate_adjoint(hipfmm_plan* plan, const void* cotangents, void* gradients, int batch,
                                  int memspace, void* stream);
 
+/* Field gradient G_cj = dH_c/dx_j at the targets (no counterpart in the
+ * reference), for gradients with respect to positions: for a loss
+ * L = sum_t g_t . H(x_t), dL/dx_t = G(t) g_t (G is symmetric), and
+ * dL/dx_s = G'(s) m_s with G' from hipfmm_plan_evaluate_field_gradient_adjoint
+ * (the field gradient at the sources of the role-exchanged problem, whose
+ * dipoles are the cotangents g).  The far part differentiates the leaf local
+ * polynomial (src/static_operators.cpp:300-335), so it is exact for the FMM's
+ * own far field and carries one order less truncation than the field; the
+ * near part is the derivative of the pair tensor (src/cuboid.cpp:293-307).
+ *   gradients: batch x target_count x 6 scalars (xx xy xz yy yz zz) in user
+ *   target order (source order for the adjoint); the rest as for
+ *   hipfmm_plan_evaluate / hipfmm_plan_evaluate_adjoint. */
+int hipfmm_plan_evaluate_field_gradient(hipfmm_plan* plan, const void* moments, void* gradients, int batch,
+                                        const int* sorted_self_indices, int memspace, void* stream);
+int hipfmm_plan_evaluate_field_gradient_adjoint(hipfmm_plan* plan, const void* cotangents, void* gradients,
+                                                int batch, int memspace, void* stream);
+
 /* Select the M2L arithmetic of later evaluates (enum hipfmm_m2l_mode). */
 int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode);
 
