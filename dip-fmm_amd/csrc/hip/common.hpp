@@ -80,6 +80,16 @@ __host__ __device__ inline void demorton3(uint32_t m, int bits, int& x, int& y, 
   z = int(compact3(m >> 2) & mask);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
+// b % 8), and each XCD has its own L2.  This bijection of [0, blocks) hands
+// XCD x one contiguous range of logical blocks instead of every 8th one, so
+// work items that share inputs (the near-field tiles of neighbouring leaves,
+// the tiles of one leaf) read them through one L2.
+__device__ inline int xcd_contiguous_block(int b, int blocks) {
+  const int q = blocks >> 3, r = blocks & 7, x = b & 7, i = b >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 template <class S>
 struct Vec4;
 template <>
@@ -112,8 +122,10 @@ struct DevicePlan {
   void* tgt_pos = nullptr;
   void* src_rel = nullptr;  // ns float4/double4 leaf-relative offsets
   void* tgt_rel = nullptr;
-  int* src_perm = nullptr;
+  int* src_perm = nullptr;  // sorted index -> user index
   int* tgt_perm = nullptr;
+  int* src_inv = nullptr;   // user index -> sorted index
+  int* tgt_inv = nullptr;
   int* self = nullptr;      // nt sorted self indices (-1 none)
   int2* leaf_src = nullptr; // per leaf [begin, end)
   int2* leaf_tgt = nullptr;

@@ -112,17 +112,58 @@ __device__ __forceinline__ void store_tile(S* __restrict__ C, int row0, int n_ro
   }
 }
 
-template <class S>
-__global__ void __launch_bounds__(kThreads) gather_kernel(const S* __restrict__ mu, const int* __restrict__ perm,
-                                                          S scale, int ns, int K, S* __restrict__ sm) {
-  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= int64_t(ns) * K) return;
-  const int j = int(idx / K), k = int(idx % K);
-  const S* src = mu + (int64_t(k) * ns + perm[j]) * 3;
-  S* dst = sm + int64_t(j) * 3 * K + k;
-  dst[0] = src[0] * scale;
-  dst[K] = src[1] * scale;
-  dst[2 * K] = src[2] * scale;
+// Moments from the user layout [k][user][3] into the sorted state layout
+// sm[sorted][3][K], times the frame factor.  A workgroup owns `points`
+// consecutive user points: it reads their rows of every state (3 * points
+// contiguous scalars per state) into LDS and writes each point's 3K scalars
+// contiguously at its sorted slot, so both sides move whole cache lines.
+// kVec: FP32, full tiles of 64 points, 16-byte aligned rows (N and K
+// multiples of 4): both sides move float4s.
+template <class S, bool kVec>
+__global__ void __launch_bounds__(kThreads) gather_kernel(const S* __restrict__ mu, const int* __restrict__ inv,
+                                                          S scale, int ns, int K, int points, S* __restrict__ sm) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  S* tile = reinterpret_cast<S*>(smem_raw);  // [point][3K + 4]
+  const int row = 3 * K + 4, width = 3 * K;
+  const int u0 = blockIdx.x * points;
+  if constexpr (kVec) {
+    constexpr int kRow4 = 3 * 64 / 4;  // float4s of one state's row of the tile
+    const float4* src = reinterpret_cast<const float4*>(mu);
+    for (int e = threadIdx.x; e < K * kRow4; e += kThreads) {
+      const int k = e / kRow4, q = e - k * kRow4;
+      const float4 v = src[(int64_t(k) * ns + u0) * 3 / 4 + q];
+      const float w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 4 * q + i, u = r / 3, c = r - 3 * u;
+        tile[u * row + c * K + k] = w[i];
+      }
+    }
+    __syncthreads();
+    const int w4 = width / 4;
+    for (int e = threadIdx.x; e < 64 * w4; e += kThreads) {
+      const int u = e / w4, q = e - u * w4;
+      float4 v = *reinterpret_cast<const float4*>(tile + u * row + 4 * q);
+      v.x *= scale;
+      v.y *= scale;
+      v.z *= scale;
+      v.w *= scale;
+      reinterpret_cast<float4*>(sm + int64_t(inv[u0 + u]) * width)[q] = v;
+    }
+  } else {
+    const int cnt = min(points, ns - u0);
+    const int span = 3 * cnt;
+    for (int e = threadIdx.x; e < span * K; e += kThreads) {
+      const int k = e / span, r = e - k * span;
+      const int u = r / 3, c = r - 3 * u;
+      tile[u * row + c * K + k] = mu[(int64_t(k) * ns + u0) * 3 + r];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < cnt * width; e += kThreads) {
+      const int u = e / width, r = e - u * width;
+      sm[int64_t(inv[u0 + u]) * width + r] = tile[u * row + r] * scale;
+    }
+  }
 }
 
 // Shared LDS layout of the monomial-based kernels (P2M, L2P).
@@ -628,20 +669,22 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 //   C_c[g][k]   = sum_s mono_g(d_s) m[s][c][k]        (|g| < p)
 // (src/static_operators.cpp:186-220); the C_c are v_mfma_f32_16x16x4_f32
 // products with the monomials as rows (A, shared by the three components),
-// 4 sources of depth per step and 2 halves of 32 states.  One workgroup (4
-// waves) per leaf and 32 states; wave w owns monomial row tiles w, w+4, ...
-// (kTW of them).  Sources go through LDS 64 at a time as axis-power tables;
-// the C_c meet in LDS, where the sign-and-shift combination writes M.
-template <int kTW>
+// 4 sources of depth per step and kH 16-state column tiles.  One workgroup
+// (4 waves) per leaf and 16 kH states; wave w owns monomial row tiles w,
+// w+4, ... (kTW of them).  Sources go through LDS 64 at a time as
+// axis-power tables; the C_c meet in LDS, where the sign-and-shift
+// combination writes M.
+template <int kTW, int kH>
 __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K) {
+  constexpr int kCols = 16 * kH;  // states of this workgroup
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = d.n, nm = d.nm, PW = d.p;  // axis powers 0..p-1
   const int NT = (nm + 15) / 16, NMP = NT * 16;
   float* pw = reinterpret_cast<float*>(smem_raw);  // [64][3][PW]
-  float* Cl = pw + 64 * 3 * PW;                      // [3][NMP][32]
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  float* Cl = pw + 64 * 3 * PW;                      // [3][NMP][kCols]
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int col = lane & 15, q = lane >> 4;
-  const int leaf = blockIdx.x, kc0 = int(blockIdx.y) * 32;
+  const int leaf = blockIdx.x, kc0 = int(blockIdx.y) * kCols;
   const int2 range = d.leaf_src[leaf];
   const int64_t node = level_offset(d.L) + leaf;
   const auto* rel = static_cast<const float4*>(d.src_rel);
@@ -654,11 +697,13 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
     row_ok[j] = g < nm;
     ex[j] = row_ok[j] ? d.p2m_tab[n + g] : int4{0, 0, 0, 0};
   }
-  f32x4 acc[kTW][3][2];
+  f32x4 acc[kTW][3][kH];
 #pragma unroll
   for (int j = 0; j < kTW; ++j)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) acc[j][c][0] = acc[j][c][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int h = 0; h < kH; ++h) acc[j][c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int j0 = range.x; j0 < range.y; j0 += 64) {
     const int cnt = min(64, range.y - j0);
     __syncthreads();  // the previous chunk's tables are no longer read
@@ -678,15 +723,17 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
       }
     }
     __syncthreads();
-    // Two steps (8 sources) per iteration: both steps' loads are issued
-    // before either's products.  Loads are unconditional (clamped index)
-    // and masked afterwards, so they do not become branches that each wait.
-    for (int s00 = 0; s00 < cnt; s00 += 8) {
-      float b[2][3][2];
-      bool okv[2];
-      int sjv[2];
+    // Four steps (16 sources) per iteration: all four steps' loads are
+    // issued before any product, so a leaf of up to 32 sources waits on
+    // memory twice.  Loads are unconditional (clamped index) and masked
+    // afterwards, so they do not become branches that each wait.
+    constexpr int kSteps = 4;
+    for (int s00 = 0; s00 < cnt; s00 += 4 * kSteps) {
+      float b[kSteps][3][kH];
+      bool okv[kSteps];
+      int sjv[kSteps];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < kSteps; ++u) {
         const int sj = s00 + 4 * u + q;
         okv[u] = sj < cnt;
         sjv[u] = okv[u] ? sj : 0;
@@ -694,10 +741,10 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int h = 0; h < 2; ++h) b[u][c][h] = ms[c * K + 16 * h];
+          for (int h = 0; h < kH; ++h) b[u][c][h] = ms[c * K + 16 * h];
       }
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < kSteps; ++u) {
         const float* ps = pw + sjv[u] * 3 * PW;
 #pragma unroll
         for (int j = 0; j < kTW; ++j) {
@@ -706,7 +753,7 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
 #pragma unroll
           for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < kH; ++h)
               acc[j][c][h] =
                   __builtin_amdgcn_mfma_f32_16x16x4f32(a, okv[u] ? b[u][c][h] : 0.0f, acc[j][c][h], 0, 0, 0);
         }
@@ -720,20 +767,20 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < kH; ++h)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          Cl[(c * NMP + (wave + 4 * j) * 16 + 4 * q + r) * 32 + 16 * h + col] = acc[j][c][h][r];
+          Cl[(c * NMP + (wave + 4 * j) * 16 + 4 * q + r) * kCols + 16 * h + col] = acc[j][c][h][r];
   }
   __syncthreads();
   float* M = static_cast<float*>(d.M) + node * int64_t(n) * K + kc0;
-  for (int w = tid; w < n * 32; w += 256) {
-    const int b = w >> 5, k = w & 31;
+  for (int w = tid; w < n * kCols; w += 256) {
+    const int b = w / kCols, k = w % kCols;
     const int4 t = d.p2m_tab[b];
     float v = 0.0f;
-    if (t.x >= 0) v += Cl[(0 * NMP + t.x) * 32 + k];
-    if (t.y >= 0) v += Cl[(1 * NMP + t.y) * 32 + k];
-    if (t.z >= 0) v += Cl[(2 * NMP + t.z) * 32 + k];
+    if (t.x >= 0) v += Cl[(0 * NMP + t.x) * kCols + k];
+    if (t.y >= 0) v += Cl[(1 * NMP + t.y) * kCols + k];
+    if (t.z >= 0) v += Cl[(2 * NMP + t.z) * kCols + k];
     M[int64_t(b) * K + k] = t.w > 0 ? v : -v;
   }
 }
@@ -754,7 +801,7 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
                                                        float* __restrict__ pot, int K) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = d.n, p = d.p, NM = kPot ? d.n : d.nm, PW = p + 1;
-  const int wave = int(threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
   float* pw = reinterpret_cast<float*>(smem_raw) + wave * (48 * PW);       // [t][axis][i]
   float* mono = reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + wave * (NM * 16);  // [g][t]
   // Basis tables shared by the workgroup: exponents of the NM monomials and
@@ -763,7 +810,7 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
   int4* up_l = ex_l + NM;
   for (int i = threadIdx.x; i < NM; i += 256) ex_l[i] = d.p2m_tab[n + i];
   for (int i = threadIdx.x; i < d.nm; i += 256) up_l[i] = d.l2p_up[i];
-  const int item = int(blockIdx.x) * 4 + wave;
+  const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * 4 + wave;
   const bool valid = item < d.p2p_tile16_count;
   const int2 tile = valid ? d.p2p_tiles16[item] : int2{0, 0};
   const int2 trange = valid ? d.leaf_tgt[tile.x] : int2{0, 0};
@@ -805,13 +852,14 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
   }
   const int nm = d.nm;
   // Loads are unconditional (past-the-end monomials read row 0 and are
-  // masked afterwards), so they do not become branches that each wait; four
-  // steps' loads are in flight together.
-  for (int g00 = 0; g00 < nm; g00 += 16) {
-    float av[4], bv[4][3][2];
-    bool okv[4];
+  // masked afterwards), so they do not become branches that each wait; eight
+  // steps' loads are in flight together (4 waits per wave at p = 8).
+  constexpr int kSteps = 8;
+  for (int g00 = 0; g00 < nm; g00 += 4 * kSteps) {
+    float av[kSteps], bv[kSteps][3][2];
+    bool okv[kSteps];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kSteps; ++u) {
       const int g = g00 + 4 * u + q;
       okv[u] = g < nm;
       const int gl = okv[u] ? g : 0;
@@ -824,7 +872,7 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
         for (int h = 0; h < 2; ++h) bv[u][c][h] = Lb[int64_t(rows[c]) * K + 16 * h];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < kSteps; ++u)
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -878,8 +926,18 @@ template <class S>
 void launch_gather(const DevicePlan& d, const S* user_moments, S scale, int K, hipStream_t s) {
   const int64_t work = int64_t(d.ns) * K;
   if (work == 0) return;
-  gather_kernel<S><<<unsigned((work + kThreads - 1) / kThreads), kThreads, 0, s>>>(
-      user_moments, d.src_perm, scale, d.ns, K, static_cast<S*>(d.sm));
+  int points = 64;  // within 64 KiB of LDS
+  while (points > 8 && size_t(points) * (3 * K + 4) * sizeof(S) > 65536) points /= 2;
+  const size_t lds = size_t(points) * (3 * K + 4) * sizeof(S);
+  const unsigned blocks = unsigned((d.ns + points - 1) / points);
+  const bool vec = sizeof(S) == 4 && points == 64 && d.ns % 64 == 0 && K % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(user_moments) % 16 == 0;
+  if (vec)
+    gather_kernel<S, true><<<blocks, kThreads, lds, s>>>(user_moments, d.src_inv, scale, d.ns, K, points,
+                                                         static_cast<S*>(d.sm));
+  else
+    gather_kernel<S, false><<<blocks, kThreads, lds, s>>>(user_moments, d.src_inv, scale, d.ns, K, points,
+                                                          static_cast<S*>(d.sm));
   HIPFMM_CHECK(hipGetLastError());
 }
 
@@ -897,14 +955,16 @@ void launch_p2m(const DevicePlan& d, int K, hipStream_t s) {
   if (d.leaves == 0) return;
   if constexpr (sizeof(S) == 4) {
     const int NT = (d.nm + 15) / 16, TW = (NT + 3) / 4;
-    const size_t lds = (size_t(64) * 3 * std::max(d.p, 1) + size_t(3) * NT * 16 * 32) * sizeof(float);
+    // 16 states per workgroup: the C_c tiles in LDS stay small enough for
+    // several workgroups per CU to hide each other's memory latency.
+    const size_t lds = (size_t(64) * 3 * std::max(d.p, 1) + size_t(3) * NT * 16 * 16) * sizeof(float);
     if (K % 32 == 0 && d.p >= 1 && TW <= 4 && lds <= 65536) {
-      const dim3 grid(unsigned(d.leaves), unsigned(K / 32));
+      const dim3 grid(unsigned(d.leaves), unsigned(K / 16));
       switch (TW) {
-        case 1: p2m_mfma_kernel<1><<<grid, 256, lds, s>>>(d, K); break;
-        case 2: p2m_mfma_kernel<2><<<grid, 256, lds, s>>>(d, K); break;
-        case 3: p2m_mfma_kernel<3><<<grid, 256, lds, s>>>(d, K); break;
-        default: p2m_mfma_kernel<4><<<grid, 256, lds, s>>>(d, K); break;
+        case 1: p2m_mfma_kernel<1, 1><<<grid, 256, lds, s>>>(d, K); break;
+        case 2: p2m_mfma_kernel<2, 1><<<grid, 256, lds, s>>>(d, K); break;
+        case 3: p2m_mfma_kernel<3, 1><<<grid, 256, lds, s>>>(d, K); break;
+        default: p2m_mfma_kernel<4, 1><<<grid, 256, lds, s>>>(d, K); break;
       }
       HIPFMM_CHECK(hipGetLastError());
       return;
@@ -929,8 +989,174 @@ void launch_l2p_grad_unsort(const DevicePlan& d, S* gradients, int K, hipStream_
   HIPFMM_CHECK(hipGetLastError());
 }
 
+// Box GEMMs on the matrix cores (FP32 plans, K a multiple of 32): M2M, L2L,
+// harmonic compression and expansion are products
+//   out[r][k] (+)= sum_{dd < D} A[dd * lda + r] in[dd][k],   r < R,
+// of a per-level operator A (depth-major, output row contiguous; rows
+// lda..R-1 read as zero) with a box's coefficient block.  One workgroup per
+// box (or parent) and 32 states: the block is staged in LDS as B[dd][48]
+// (rows >= D zero; the 48-float stride keeps the four depth rows of a
+// v_mfma_f32_16x16x4_f32 B fragment on distinct banks), and wave w owns the
+// 16-state column tile w & 1 and the row tiles (w >> 1) + 2t.  Depth steps
+// of 4 go in batches of 7 (21 and 42 steps at p = 8), each batch's operand
+// loads issued before its products, so a wave waits on the operator's L2
+// latency once per 28 depth rows; clamped unconditional loads, zero-selected
+// past D and lda.
+constexpr int kBoxLd = 48;
+
+__device__ __forceinline__ void stage_box_mfma(float* __restrict__ Bl, const float* __restrict__ in, int D, int D4,
+                                               int K) {
+  for (int w = threadIdx.x; w < D4 * 8; w += 256) {
+    const int dd = w >> 3, v = w & 7;
+    float4 x = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (dd < D) x = *reinterpret_cast<const float4*>(in + int64_t(dd) * K + 4 * v);
+    *reinterpret_cast<float4*>(Bl + dd * kBoxLd + 4 * v) = x;
+  }
+}
+
+template <int RTW>
+__device__ __forceinline__ void box_mfma_tiles(const float* __restrict__ A, int lda, int D, int RT,
+                                               const float* __restrict__ Bl, int wave, int lane, f32x4 (&acc)[RTW]) {
+  const int i = lane & 15, q = lane >> 4, ct = wave & 1, rt0 = wave >> 1;
+  constexpr int kBatch = 7;
+  const int steps = (D + 3) / 4;
+  for (int s0 = 0; s0 < steps; s0 += kBatch) {
+    float a[kBatch][RTW], b[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int dd = 4 * (s0 + u) + q;
+      const bool dok = dd < D;
+      const int ddc = dok ? dd : D - 1;
+      b[u] = Bl[(dok ? dd : 0) * kBoxLd + 16 * ct + i];
+      b[u] = dok ? b[u] : 0.0f;
+#pragma unroll
+      for (int t = 0; t < RTW; ++t) {
+        const int row = (rt0 + 2 * t) * 16 + i;
+        const bool ok = row < lda;
+        const float v = A[int64_t(ddc) * lda + (ok ? row : 0)];
+        a[u][t] = ok ? v : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+#pragma unroll
+      for (int t = 0; t < RTW; ++t)
+        if (rt0 + 2 * t < RT) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u][t], b[u], acc[t], 0, 0, 0);
+  }
+}
+
+// Rows [R) of the wave's tiles to out[row][K] (out offset to the first state).
+template <int RTW, bool kAdd>
+__device__ __forceinline__ void box_mfma_store(float* __restrict__ out, int R, int RT, int K, int wave, int lane,
+                                               const f32x4 (&acc)[RTW]) {
+  const int i = lane & 15, q = lane >> 4, ct = wave & 1, rt0 = wave >> 1;
+#pragma unroll
+  for (int t = 0; t < RTW; ++t) {
+    if (rt0 + 2 * t >= RT) break;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = (rt0 + 2 * t) * 16 + 4 * q + r;
+      if (row >= R) continue;
+      float* o = out + int64_t(row) * K + 16 * ct + i;
+      if constexpr (kAdd)
+        *o += acc[t][r];
+      else
+        *o = acc[t][r];
+    }
+  }
+}
+
+// Single-box products (harmonic compression, expansion): out = A in.
+template <int RTW>
+__global__ void __launch_bounds__(256) box_mfma_kernel(const float* __restrict__ A, int lda, int R, int D,
+                                                       const float* __restrict__ in, int64_t in_stride,
+                                                       float* __restrict__ out, int64_t out_stride, int K) {
+  extern __shared__ __align__(16) float box_lds[];
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  const int kc0 = int(blockIdx.y) * 32, D4 = (D + 3) / 4 * 4, RT = (R + 15) / 16;
+  stage_box_mfma(box_lds, in + int64_t(blockIdx.x) * in_stride + kc0, D, D4, K);
+  __syncthreads();
+  f32x4 acc[RTW];
+#pragma unroll
+  for (int t = 0; t < RTW; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  box_mfma_tiles<RTW>(A, lda, D, RT, box_lds, wave, lane, acc);
+  box_mfma_store<RTW, false>(out + int64_t(blockIdx.x) * out_stride + kc0, R, RT, K, wave, lane, acc);
+}
+
+// M2M: parent (n coefficients) = sum over its 8 children c of T_c child.
+template <int RTW>
+__global__ void __launch_bounds__(256) m2m_mfma_kernel(const DevicePlan d, int level, int K) {
+  extern __shared__ __align__(16) float box_lds[];
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  const int n = d.n, NP = round_up(n, kRA), D4 = (n + 3) / 4 * 4, RT = (n + 15) / 16;
+  const int kc0 = int(blockIdx.y) * 32;
+  const int64_t parent = blockIdx.x, cnode0 = level_offset(level + 1) + parent * 8;
+  const float* M = static_cast<const float*>(d.M) + kc0;
+  f32x4 acc[RTW];
+#pragma unroll
+  for (int t = 0; t < RTW; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int c = 0; c < 8; ++c) {
+    __syncthreads();
+    stage_box_mfma(box_lds, M + (cnode0 + c) * int64_t(n) * K, n, D4, K);
+    __syncthreads();
+    box_mfma_tiles<RTW>(shift_operator<float>(d.m2m_dense, level + 1, c, n, NP), NP, n, RT, box_lds, wave, lane, acc);
+  }
+  box_mfma_store<RTW, false>(static_cast<float*>(d.M) + (level_offset(level) + parent) * int64_t(n) * K + kc0, n, RT,
+                             K, wave, lane, acc);
+}
+
+// L2L: child c += T_c parent, for the 8 children of one parent.
+template <int RTW>
+__global__ void __launch_bounds__(256) l2l_mfma_kernel(const DevicePlan d, int child_level, int K) {
+  extern __shared__ __align__(16) float box_lds[];
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  const int n = d.n, NP = round_up(n, kRA), D4 = (n + 3) / 4 * 4, RT = (n + 15) / 16;
+  const int kc0 = int(blockIdx.y) * 32;
+  const int64_t parent = blockIdx.x;
+  float* Lc = static_cast<float*>(d.Lc) + kc0;
+  stage_box_mfma(box_lds, Lc + (level_offset(child_level - 1) + parent) * int64_t(n) * K, n, D4, K);
+  __syncthreads();
+  for (int c = 0; c < 8; ++c) {
+    f32x4 acc[RTW];
+#pragma unroll
+    for (int t = 0; t < RTW; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    box_mfma_tiles<RTW>(shift_operator<float>(d.l2l_dense, child_level, c, n, NP), NP, n, RT, box_lds, wave, lane,
+                        acc);
+    const int64_t child = level_offset(child_level) + parent * 8 + c;
+    box_mfma_store<RTW, true>(Lc + child * int64_t(n) * K, n, RT, K, wave, lane, acc);
+  }
+}
+
+// Row tiles per wave of the box products, or 0 where they do not apply.
+inline int box_mfma_rtw(int R, int D, int K) {
+  const int rtw = ((R + 15) / 16 + 1) / 2;
+  if (K % 32 != 0 || rtw > 6 || size_t((D + 3) / 4 * 4) * kBoxLd * sizeof(float) > 65536) return 0;
+  return rtw;
+}
+
+#define HIPFMM_BOX_RTW(rtw, launch) \
+  switch (rtw) {                    \
+    case 1: launch(1); break;       \
+    case 2: launch(2); break;       \
+    case 3: launch(3); break;       \
+    case 4: launch(4); break;       \
+    case 5: launch(5); break;       \
+    default: launch(6); break;      \
+  }
+
 template <class S>
 void launch_m2m(const DevicePlan& d, int parent_level, int K, hipStream_t s) {
+  if constexpr (sizeof(S) == 4) {
+    if (const int rtw = box_mfma_rtw(d.n, d.n, K)) {
+      const dim3 grid(unsigned(int64_t(1) << (3 * parent_level)), unsigned(K / 32));
+      const size_t lds = size_t((d.n + 3) / 4 * 4) * kBoxLd * sizeof(float);
+#define HIPFMM_M2M(R) m2m_mfma_kernel<R><<<grid, 256, lds, s>>>(d, parent_level, K)
+      HIPFMM_BOX_RTW(rtw, HIPFMM_M2M)
+#undef HIPFMM_M2M
+      HIPFMM_CHECK(hipGetLastError());
+      return;
+    }
+  }
   const int NP = round_up(d.n, kRA);
   const int kc = state_chunk(K, [&](int c) { return size_t(d.n) * round_up(c, kRK) * sizeof(S); });
   const int KP = round_up(kc, kRK);
@@ -941,6 +1167,17 @@ void launch_m2m(const DevicePlan& d, int parent_level, int K, hipStream_t s) {
 
 template <class S>
 void launch_l2l(const DevicePlan& d, int child_level, int K, hipStream_t s) {
+  if constexpr (sizeof(S) == 4) {
+    if (const int rtw = box_mfma_rtw(d.n, d.n, K)) {
+      const dim3 grid(unsigned(int64_t(1) << (3 * (child_level - 1))), unsigned(K / 32));
+      const size_t lds = size_t((d.n + 3) / 4 * 4) * kBoxLd * sizeof(float);
+#define HIPFMM_L2L(R) l2l_mfma_kernel<R><<<grid, 256, lds, s>>>(d, child_level, K)
+      HIPFMM_BOX_RTW(rtw, HIPFMM_L2L)
+#undef HIPFMM_L2L
+      HIPFMM_CHECK(hipGetLastError());
+      return;
+    }
+  }
   const int NP = round_up(d.n, kRA);
   const int kc = state_chunk(K, [&](int c) { return size_t(d.n) * round_up(c, kRK) * sizeof(S); });
   const int KP = round_up(kc, kRK);
@@ -995,6 +1232,20 @@ void launch_m2l_compress(const DevicePlan& d, int K, bool split, int first_level
         HIPFMM_CHECK(hipGetLastError());
         continue;
       }
+      if (const int rtw = box_mfma_rtw(d.nc, d.n, K)) {
+        const dim3 g2(grid.x, unsigned(K / 32));
+        const size_t l2 = size_t((d.n + 3) / 4 * 4) * kBoxLd * sizeof(float);
+        const float* A = static_cast<const float*>(d.cmp_op) + size_t(level) * d.n * NPc;
+        const int64_t node0 = level_offset(level);
+        const float* in = static_cast<const float*>(d.M) + node0 * d.n * K;
+        float* out = static_cast<float*>(d.Mc) + node0 * d.nc * K;
+#define HIPFMM_CMP(R) box_mfma_kernel<R><<<g2, 256, l2, s>>>(A, NPc, d.nc, d.n, in, int64_t(d.n) * K, out, \
+                                                              int64_t(d.nc) * K, K)
+        HIPFMM_BOX_RTW(rtw, HIPFMM_CMP)
+#undef HIPFMM_CMP
+        HIPFMM_CHECK(hipGetLastError());
+        continue;
+      }
     }
     if (split) throw invalid_argument("the split compressed layout is an FP32 operand");
     compress_kernel<S, false><<<grid, threads, lds, s>>>(d, level, K, kc, KP);
@@ -1009,6 +1260,22 @@ void launch_m2l_expand(const DevicePlan& d, int K, int first_level, hipStream_t 
   const int KP = round_up(kc, kRK);
   for (int level = std::max(first_level, 2); level <= d.L; ++level) {
     const dim3 grid(unsigned(int64_t(1) << (3 * level)), unsigned((K + kc - 1) / kc));
+    if constexpr (sizeof(S) == 4) {
+      if (const int rtw = box_mfma_rtw(d.n, d.nc, K)) {
+        const dim3 g2(grid.x, unsigned(K / 32));
+        const size_t l2 = size_t((d.nc + 3) / 4 * 4) * kBoxLd * sizeof(float);
+        const float* A = static_cast<const float*>(d.exp_op) + size_t(level) * d.nc * NP;
+        const int64_t node0 = level_offset(level);
+        const float* in = static_cast<const float*>(d.Lcc) + node0 * d.nc * K;
+        float* out = static_cast<float*>(d.Lc) + node0 * d.n * K;
+#define HIPFMM_EXP(R) box_mfma_kernel<R><<<g2, 256, l2, s>>>(A, NP, d.n, d.nc, in, int64_t(d.nc) * K, out, \
+                                                              int64_t(d.n) * K, K)
+        HIPFMM_BOX_RTW(rtw, HIPFMM_EXP)
+#undef HIPFMM_EXP
+        HIPFMM_CHECK(hipGetLastError());
+        continue;
+      }
+    }
     expand_kernel<S><<<grid, tile_threads(NP / kRA, KP), size_t(d.nc) * KP * sizeof(S), s>>>(d, level, K, kc, KP);
     HIPFMM_CHECK(hipGetLastError());
   }

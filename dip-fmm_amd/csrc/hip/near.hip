@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(kThreads) p2p_kernel(const DevicePlan d, int K
   auto* spos = reinterpret_cast<PairPos<S>*>(smem_raw);       // chunk
   S* smom = reinterpret_cast<S*>(spos + chunk);                // chunk x 3 x K
   S* red = smom;                                               // reused for the split reduction
-  const int leaf = blockIdx.x;
+  const int leaf = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x));
   const int2 trange = d.leaf_tgt[leaf];
   if (trange.x == trange.y) return;
   const int slot = threadIdx.x % kSlots, g = threadIdx.x / kSlots;
@@ -159,7 +159,9 @@ void launch_p2p_t(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   while (kg < KG) kg *= 2;  // 1, 2, 4 or 8 k groups
   const int chunk = sizeof(S) == 4 ? 64 : 32;
   const size_t stage = size_t(chunk) * sizeof(PairPos<S>) + size_t(chunk) * 3 * K * sizeof(S);
-  const size_t reduce = size_t(8) * kSlots * KPT * 4 * sizeof(S) + size_t(chunk) * sizeof(PairPos<S>);
+  // The cross-group reduction runs only when several groups share a state
+  // group (kg < 8).
+  const size_t reduce = kg < 8 ? size_t(8) * kSlots * KPT * 4 * sizeof(S) + size_t(chunk) * sizeof(PairPos<S>) : 0;
   if (potential)
     p2p_kernel<S, KPT, true><<<unsigned(d.leaves), kThreads, std::max(stage, reduce), s>>>(d, K, kg, chunk);
   else
@@ -234,8 +236,8 @@ constexpr int kP2PWaves = 4;
 template <int kRows>
 __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma_kernel(const DevicePlan d, const int2* __restrict__ tiles,
                                                                    int tile_count, int K) {
-  const int wave = int(threadIdx.x) >> 6, lane = threadIdx.x & 63;
-  const int item = int(blockIdx.x) * kP2PWaves + wave;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * kP2PWaves + wave;
   if (item >= tile_count) return;
   const int2 tile = tiles[item];
   const int leaf = tile.x;
@@ -346,11 +348,19 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // reciprocal square root (1 ulp).
 // One step of the targets-as-rows kernel for this lane's pair (row target,
 // source s), in two stages so that several steps' loads are in flight
-// before their products: the raw loads (source position, moments m[s][c'][k] of both
-// 16-state halves; the index is clamped to `first` so every load is in the
-// leaf and unconditional), then the operands -- the 6 distinct entries of D
-// (d00 d01 d02 d11 d12 d22), the potential row f r, and the masked moments.
-// Sources at or past `end` and the identity pair give zeros.
+// before their products: the raw loads (source position, moments m[s][c'][k]
+// of both 16-state halves), then the operands -- the 6 distinct entries of
+// D (d00 d01 d02 d11 d12 d22) and the potential row f r.  Loads address the
+// neighbour leaf's sources from wave-uniform bases with a 32-bit lane
+// offset; a slot past the leaf's end reads its first source.  Padding slots
+// and the identity pair get |r|^-1 = 0, so their D and f r entries are 0 and
+// the moments they meet need no masking (a finite moment times 0).
+struct P2PSources {
+  const float4* hi;  // hi and lo blocks of the leaf's first source position
+  const float4* lo;
+  const float* m;    // m[first][0][kc0]
+};
+
 struct P2PRaw {
   PairPos<float> sp;
   float b[3][2];
@@ -360,19 +370,18 @@ struct P2PRaw {
 struct P2POperands {
   float d[6];
   float p[3];
-  float b[3][2];
 };
 
-__device__ __forceinline__ P2PRaw p2p_load(const DevicePlan& d, const float* __restrict__ sm, int s, int end,
-                                           int first, int K, int kcol) {
+__device__ __forceinline__ P2PRaw p2p_load(const P2PSources& src, int s, int first, int end, int K, int col) {
   P2PRaw r;
-  const int sl = s < end ? s : first;
-  r.sp = PairPos<float>::load(d.src_pos, d.ns, sl);
-  const float* ms = sm + int64_t(sl) * 3 * K + kcol;
+  const uint32_t j = s < end ? uint32_t(s - first) : 0u;
+  r.sp.hi = src.hi[j];
+  r.sp.lo = src.lo[j];
+  const uint32_t off = j * uint32_t(3 * K) + uint32_t(col);
 #pragma unroll
   for (int cp = 0; cp < 3; ++cp)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) r.b[cp][h] = ms[cp * K + 16 * h];
+    for (int h = 0; h < 2; ++h) r.b[cp][h] = src.m[off + uint32_t(cp * K + 16 * h)];
   r.s = s;
   return r;
 }
@@ -381,40 +390,68 @@ template <int kComps>
 __device__ __forceinline__ P2POperands p2p_finish(const P2PRaw& r, const PairPos<float>& tp, bool row_ok, int self,
                                                   int end) {
   P2POperands o;
-  const bool src_ok = r.s < end;
   float rx, ry, rz;
   tp.diff(r.sp, rx, ry, rz);
-#pragma unroll
-  for (int cp = 0; cp < 3; ++cp)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) o.b[cp][h] = src_ok ? r.b[cp][h] : 0.0f;
   const float r2 = rx * rx + ry * ry + rz * rz;
-  const float rinv = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32 (1 ulp); r = 0 gives inf, hence NaN below
+  const bool use = row_ok & (r.s < end) & (r.s != self);
+  // v_rsq_f32 (1 ulp); a coincident pair (r = 0) that is used gives inf,
+  // hence NaN entries below.
+  const float rinv = use ? __builtin_amdgcn_rsqf(r2) : 0.0f;
   const float f = rinv * rinv * rinv * 0.079577471545947667884441881686257181f;
   const float ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
-  const bool use = row_ok & src_ok & (r.s != self);
   const float f3 = 3.0f * f;
-  o.d[0] = use ? f3 * ux * ux - f : 0.0f;
-  o.d[1] = use ? f3 * ux * uy : 0.0f;
-  o.d[2] = use ? f3 * ux * uz : 0.0f;
-  o.d[3] = use ? f3 * uy * uy - f : 0.0f;
-  o.d[4] = use ? f3 * uy * uz : 0.0f;
-  o.d[5] = use ? f3 * uz * uz - f : 0.0f;
+  o.d[0] = f3 * ux * ux - f;
+  o.d[1] = f3 * ux * uy;
+  o.d[2] = f3 * ux * uz;
+  o.d[3] = f3 * uy * uy - f;
+  o.d[4] = f3 * uy * uz;
+  o.d[5] = f3 * uz * uz - f;
   if constexpr (kComps == 4) {
-    o.p[0] = use ? f * rx : 0.0f;
-    o.p[1] = use ? f * ry : 0.0f;
-    o.p[2] = use ? f * rz : 0.0f;
+    o.p[0] = f * rx;
+    o.p[1] = f * ry;
+    o.p[2] = f * rz;
   } else {
     o.p[0] = o.p[1] = o.p[2] = 0.0f;
   }
   return o;
 }
 
+// kU steps of 4 sources from j00: all loads, then geometry and products.
+template <int kComps, int kU>
+__device__ __forceinline__ void p2p_steps(const P2PSources& src, int j00, int q, int end, int first, int K, int col,
+                                          const PairPos<float>& tp, bool row_ok, int self,
+                                          f32x4 (&part)[kComps][2]) {
+  P2PRaw raw[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) raw[u] = p2p_load(src, j00 + 4 * u + q, first, end, K, col);
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const P2POperands cur = p2p_finish<kComps>(raw[u], tp, row_ok, self, end);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[0], raw[u].b[0][h], part[0][h], 0, 0, 0);
+      part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], raw[u].b[0][h], part[1][h], 0, 0, 0);
+      part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], raw[u].b[0][h], part[2][h], 0, 0, 0);
+      part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], raw[u].b[1][h], part[0][h], 0, 0, 0);
+      part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[3], raw[u].b[1][h], part[1][h], 0, 0, 0);
+      part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], raw[u].b[1][h], part[2][h], 0, 0, 0);
+      part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], raw[u].b[2][h], part[0][h], 0, 0, 0);
+      part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], raw[u].b[2][h], part[1][h], 0, 0, 0);
+      part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[5], raw[u].b[2][h], part[2][h], 0, 0, 0);
+      if constexpr (kComps == 4) {
+        part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[0], raw[u].b[0][h], part[3][h], 0, 0, 0);
+        part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[1], raw[u].b[1][h], part[3][h], 0, 0, 0);
+        part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[2], raw[u].b[2][h], part[3][h], 0, 0, 0);
+      }
+    }
+  }
+}
+
 template <int kComps>
 __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
                                                                      int tile_count, int K) {
-  const int wave = int(threadIdx.x) >> 6, lane = threadIdx.x & 63;
-  const int item = int(blockIdx.x) * kP2PWaves + wave;
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * kP2PWaves + wave;
   if (item >= tile_count) return;
   const int2 tile = tiles[item];
   const int leaf = tile.x;
@@ -440,6 +477,8 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const Device
         const int x = lx + dx, y = ly + dy, z = lz + dz;
         if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
         const int2 sr = d.leaf_src[morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L)];
+        const P2PSources src{static_cast<const float4*>(d.src_pos) + sr.x,
+                             static_cast<const float4*>(d.src_pos) + d.ns + sr.x, sm + int64_t(sr.x) * 3 * K + kc0};
         for (int jb = sr.x; jb < sr.y; jb += 32) {
           const int je = min(sr.y, jb + 32);
           f32x4 part[kComps][2];
@@ -447,33 +486,14 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const Device
           for (int c = 0; c < kComps; ++c)
 #pragma unroll
             for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-          // Two steps (8 sources) per iteration: both steps' loads are
-          // issued before either's pair geometry and products.
+          // Two steps (8 sources) per iteration, both steps' loads issued
+          // before either's pair geometry and products; a block's last 1-4
+          // sources take one step.
           for (int j00 = jb; j00 < je; j00 += 8) {
-            P2PRaw raw[2];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) raw[u] = p2p_load(d, sm, j00 + 4 * u + q, je, sr.x, K, kc0 + col);
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-              const P2POperands cur = p2p_finish<kComps>(raw[u], tp, row_ok, self, je);
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[0], cur.b[0][h], part[0][h], 0, 0, 0);
-                part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], cur.b[0][h], part[1][h], 0, 0, 0);
-                part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], cur.b[0][h], part[2][h], 0, 0, 0);
-                part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[1], cur.b[1][h], part[0][h], 0, 0, 0);
-                part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[3], cur.b[1][h], part[1][h], 0, 0, 0);
-                part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], cur.b[1][h], part[2][h], 0, 0, 0);
-                part[0][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[2], cur.b[2][h], part[0][h], 0, 0, 0);
-                part[1][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[4], cur.b[2][h], part[1][h], 0, 0, 0);
-                part[2][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.d[5], cur.b[2][h], part[2][h], 0, 0, 0);
-                if constexpr (kComps == 4) {
-                  part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[0], cur.b[0][h], part[3][h], 0, 0, 0);
-                  part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[1], cur.b[1][h], part[3][h], 0, 0, 0);
-                  part[3][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.p[2], cur.b[2][h], part[3][h], 0, 0, 0);
-                }
-              }
-            }
+            if (je - j00 > 4)
+              p2p_steps<kComps, 2>(src, j00, q, je, sr.x, K, col, tp, row_ok, self, part);
+            else
+              p2p_steps<kComps, 1>(src, j00, q, je, sr.x, K, col, tp, row_ok, self, part);
           }
 #pragma unroll
           for (int c = 0; c < kComps; ++c)
@@ -550,7 +570,7 @@ __global__ void __launch_bounds__(kThreads) p2p_grad_kernel(const DevicePlan d, 
   extern __shared__ unsigned char smem_raw[];
   auto* spos = reinterpret_cast<PairPos<S>*>(smem_raw);  // chunk
   S* smom = reinterpret_cast<S*>(spos + chunk);           // chunk x 3 x K
-  const int leaf = blockIdx.x;
+  const int leaf = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x));
   const int2 trange = d.leaf_tgt[leaf];
   if (trange.x == trange.y) return;
   const int slot = threadIdx.x % kSlots, g = threadIdx.x / kSlots;
@@ -640,7 +660,7 @@ void launch_p2p_grad_t(const DevicePlan& d, int K, hipStream_t s) {
   while (kg < KG) kg *= 2;
   const int chunk = sizeof(S) == 4 ? 64 : 32;
   const size_t stage = size_t(chunk) * sizeof(PairPos<S>) + size_t(chunk) * 3 * K * sizeof(S);
-  const size_t reduce = size_t(8) * kSlots * KPT * 6 * sizeof(S) + size_t(chunk) * sizeof(PairPos<S>);
+  const size_t reduce = kg < 8 ? size_t(8) * kSlots * KPT * 6 * sizeof(S) + size_t(chunk) * sizeof(PairPos<S>) : 0;
   p2p_grad_kernel<S, KPT><<<unsigned(d.leaves), kThreads, std::max(stage, reduce), s>>>(d, K, kg, chunk);
   HIPFMM_CHECK(hipGetLastError());
 }

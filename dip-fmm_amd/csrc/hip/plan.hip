@@ -312,6 +312,16 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   pack(desc.sorted_targets, d.nt, d.tgt_pos, d.tgt_rel, false);
   d.src_perm = upload(P, desc.source_permutation, std::max(d.ns, 1));
   d.tgt_perm = upload(P, desc.target_permutation, std::max(d.nt, 1));
+  auto inverse = [&](const int* perm, int count, const char* what) {
+    std::vector<int> inv(std::max(count, 1), -1);
+    for (int j = 0; j < count; ++j) {
+      require(perm[j] >= 0 && perm[j] < count && inv[perm[j]] < 0, what);
+      inv[perm[j]] = j;
+    }
+    return upload(P, inv.data(), inv.size());
+  };
+  d.src_inv = inverse(desc.source_permutation, d.ns, "source_permutation is not a permutation");
+  d.tgt_inv = inverse(desc.target_permutation, d.nt, "target_permutation is not a permutation");
   std::vector<int2> ls(d.leaves), lt(d.leaves);
   for (int64_t leaf = 0; leaf < d.leaves; ++leaf) {
     const int64_t* r = ranges + 4 * (leaf0 + leaf);
@@ -750,6 +760,7 @@ DevicePlan role_exchanged(const hipfmm_plan* P) {
   std::swap(a.src_pos, a.tgt_pos);
   std::swap(a.src_rel, a.tgt_rel);
   std::swap(a.src_perm, a.tgt_perm);
+  std::swap(a.src_inv, a.tgt_inv);
   std::swap(a.leaf_src, a.leaf_tgt);
   a.self = P->adj_self;
   a.p2p_tiles = P->adj_tiles;
