@@ -770,14 +770,31 @@ DevicePlan role_exchanged(const hipfmm_plan* P) {
   return a;
 }
 
-// One evaluate enqueued on the plan's streams.  With `capture` the caller
-// is recording a hipGraph on `user`: state is already allocated, only the
-// events that fork and join the streams are recorded (no timing events),
-// and nothing waits on the host.
+// Identity map: fixed with the plan on first use (src/cuda_fmm.cu:3470-3486);
+// null keeps the plan's map.
+void install_identity(hipfmm_plan* P, const int* self) {
+  if (self == nullptr) return;
+  const hipfmm::DevicePlan& f = P->d;
+  if (!P->identity_initialised) {
+    for (int t = 0; t < f.nt; ++t) require(self[t] >= -1 && self[t] < f.ns, "identity map contains an invalid index");
+    P->fixed_self.assign(self, self + f.nt);
+    if (f.nt) HIPFMM_CHECK(hipMemcpy(f.self, self, size_t(f.nt) * sizeof(int), hipMemcpyHostToDevice));
+    P->stats.setup_h2d_bytes += size_t(f.nt) * sizeof(int);
+    P->identity_initialised = true;
+    refresh_adjoint_self(P);
+  } else if (!std::equal(self, self + f.nt, P->fixed_self.begin())) {
+    throw hipfmm::invalid_argument("HipFull identity map changed; rebuild the static plan");
+  }
+}
+
+// One evaluate enqueued on the plan's streams.  `fork`: when set, the
+// plan's streams start after this event (recorded by the caller on `user`)
+// instead of an event of their own; `join` = false leaves the wait of `user`
+// on the plan's last stage to the caller.
 template <class S>
 void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* potentials, int K, const int* self,
-                   int memspace, hipStream_t user, bool adjoint, bool capture = false,
-                   void* field_gradients = nullptr) {
+                   int memspace, hipStream_t user, bool adjoint, void* field_gradients = nullptr,
+                   hipEvent_t fork = nullptr, bool join = true) {
   const bool want_pot = potentials != nullptr;
   const bool want_grad = field_gradients != nullptr;
   {
@@ -791,38 +808,19 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     require(want_pot || want_grad || fields != nullptr || n_out == 0,
             adjoint ? "gradients pointer is null" : "fields pointer is null");
     require(memspace == HIPFMM_HOST || memspace == HIPFMM_DEVICE, "unknown memory space");
-    // Identity map: fixed with the plan on first use (src/cuda_fmm.cu:3470-3486).
-    if (self != nullptr) {
-      if (!P->identity_initialised) {
-        for (int t = 0; t < f.nt; ++t)
-          require(self[t] >= -1 && self[t] < f.ns, "identity map contains an invalid index");
-        P->fixed_self.assign(self, self + f.nt);
-        if (f.nt) HIPFMM_CHECK(hipMemcpy(f.self, self, size_t(f.nt) * sizeof(int), hipMemcpyHostToDevice));
-        P->stats.setup_h2d_bytes += size_t(f.nt) * sizeof(int);
-        P->identity_initialised = true;
-        refresh_adjoint_self(P);
-      } else if (!std::equal(self, self + f.nt, P->fixed_self.begin())) {
-        throw hipfmm::invalid_argument("HipFull identity map changed; rebuild the static plan");
-      }
-    }
+    install_identity(P, self);
     require(!adjoint || P->adj_self_valid,
             "the identity map pairs two targets with one source; the adjoint has no transposed form of it");
   }
-  require(!capture || memspace == HIPFMM_DEVICE, "a captured evaluate takes device operands");
-  if (!capture) {
-    // No wait on earlier evaluates here: their stage events are read back
-    // only when timings are asked for, so the host can enqueue ahead.
-    (void)hipGetLastError();
-    ensure_capacity<S>(P, K, want_pot, want_grad);
-  } else {
-    require(K <= P->d.kcap && (!want_pot || P->potential_used) && (!want_grad || P->gradient_used),
-            "state is not allocated for a captured evaluate");
-  }
+  // No wait on earlier evaluates here: their stage events are read back
+  // only when timings are asked for, so the host can enqueue ahead.
+  (void)hipGetLastError();
+  ensure_capacity<S>(P, K, want_pot, want_grad);
   const DevicePlan d = adjoint ? role_exchanged(P) : P->d;
   hipStream_t s = P->s_far;
   hipEvent_t* e = P->ev;
   auto mark = [&](int ev, hipStream_t on) {
-    if (!capture) HIPFMM_CHECK(hipEventRecord(e[ev], on));
+    HIPFMM_CHECK(hipEventRecord(e[ev], on));
   };
   const S* mu = nullptr;
   S* out = nullptr;
@@ -872,8 +870,11 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     P->stats.evaluation_h2d_calls += 1;
     P->stats.evaluation_d2h_calls += 1;
   } else {
-    HIPFMM_CHECK(hipEventRecord(e[kEvUser], user));
-    HIPFMM_CHECK(hipStreamWaitEvent(s, e[kEvUser], 0));
+    if (fork == nullptr) {
+      HIPFMM_CHECK(hipEventRecord(e[kEvUser], user));
+      fork = e[kEvUser];
+    }
+    HIPFMM_CHECK(hipStreamWaitEvent(s, fork, 0));
     mark(kEvH2D0, s);
     mark(kEvH2D1, s);
     mu = static_cast<const S*>(moments);
@@ -942,20 +943,16 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   }
   mark(kEvM2LKernel0, s);
   const int ring_slot = int(P->m2l_launch_count % hipfmm_plan::kRing);
-  if (!capture) {
-    drain_m2l_slot(P, ring_slot);
-    HIPFMM_CHECK(hipEventRecord(P->m2l_ring[ring_slot][0], s));
-  }
+  drain_m2l_slot(P, ring_slot);
+  HIPFMM_CHECK(hipEventRecord(P->m2l_ring[ring_slot][0], s));
   if (mfma)
     hipfmm::launch_m2l_mfma(dm, K, P->m2l_mode, s);
   else
     hipfmm::launch_m2l_valu<S>(dm, K, s);
   mark(kEvM2LKernel1, s);
-  if (!capture) {
-    HIPFMM_CHECK(hipEventRecord(P->m2l_ring[ring_slot][1], s));
-    P->m2l_ring_pending[ring_slot] = true;
-    ++P->m2l_launch_count;
-  }
+  HIPFMM_CHECK(hipEventRecord(P->m2l_ring[ring_slot][1], s));
+  P->m2l_ring_pending[ring_slot] = true;
+  ++P->m2l_launch_count;
   if (d.nc > 0 && d.L >= 2 && !harmonic) hipfmm::launch_m2l_expand<S>(d, K, 2, s);
   mark(kEvM2L, s);
   if (harmonic) {
@@ -973,7 +970,7 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
   HIPFMM_CHECK(hipEventRecord(e[kEvL2P], s));
   P->last_K = K;
   P->last_host = host;
-  P->timings_pending = !capture;
+  P->timings_pending = true;
   if (host) {
     HIPFMM_CHECK(hipEventRecord(e[kEvD2H0], s));
     if (out_bytes) HIPFMM_CHECK(hipMemcpyAsync(P->pinned_out, d.fields_out, out_bytes, hipMemcpyDeviceToHost, s));
@@ -986,7 +983,7 @@ void evaluate_impl(hipfmm_plan* P, const void* moments, void* fields, void* pote
     if (pot_bytes) std::memcpy(potentials, P->pinned_pot, pot_bytes);
     if (grad_bytes) std::memcpy(field_gradients, P->pinned_grad, grad_bytes);
     finalize_timings(P);
-  } else {
+  } else if (join) {
     HIPFMM_CHECK(hipStreamWaitEvent(user, e[kEvL2P], 0));
   }
 }
@@ -1114,10 +1111,10 @@ int hipfmm_plan_evaluate_field_gradient(hipfmm_plan* plan, const void* moments, 
     DeviceGuard guard(plan->device);
     const hipStream_t user = static_cast<hipStream_t>(stream);
     if (plan->d.precision == 32)
-      evaluate_impl<float>(plan, moments, nullptr, nullptr, batch, sorted_self_indices, memspace, user, false, false,
+      evaluate_impl<float>(plan, moments, nullptr, nullptr, batch, sorted_self_indices, memspace, user, false,
                            gradients);
     else
-      evaluate_impl<double>(plan, moments, nullptr, nullptr, batch, sorted_self_indices, memspace, user, false, false,
+      evaluate_impl<double>(plan, moments, nullptr, nullptr, batch, sorted_self_indices, memspace, user, false,
                             gradients);
   });
 }
@@ -1131,9 +1128,9 @@ int hipfmm_plan_evaluate_field_gradient_adjoint(hipfmm_plan* plan, const void* c
     DeviceGuard guard(plan->device);
     const hipStream_t user = static_cast<hipStream_t>(stream);
     if (plan->d.precision == 32)
-      evaluate_impl<float>(plan, cotangents, nullptr, nullptr, batch, nullptr, memspace, user, true, false, gradients);
+      evaluate_impl<float>(plan, cotangents, nullptr, nullptr, batch, nullptr, memspace, user, true, gradients);
     else
-      evaluate_impl<double>(plan, cotangents, nullptr, nullptr, batch, nullptr, memspace, user, true, false, gradients);
+      evaluate_impl<double>(plan, cotangents, nullptr, nullptr, batch, nullptr, memspace, user, true, gradients);
   });
 }
 
@@ -1335,15 +1332,15 @@ int hipfmm_direct_field(const double* targets, int nt, const double* sources, in
 
 struct hipfmm_group {
   std::vector<hipfmm_plan*> plans;
-  hipStream_t capture = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipEvent_t done = nullptr;
+  std::vector<const void*> moments;
+  std::vector<void*> fields;
+  std::vector<int> batches;
+  hipEvent_t fork = nullptr;
   int device = 0;
 };
 
 int hipfmm_group_create(hipfmm_plan* const* plans, int count, const void* const* moments, void* const* fields,
-                        const int* batches, hipfmm_group** group) {
+                        const int* batches, const int* const* sorted_self_indices, hipfmm_group** group) {
   return guarded([&] {
     require(group != nullptr, "group output is null");
     *group = nullptr;
@@ -1354,47 +1351,28 @@ int hipfmm_group_create(hipfmm_plan* const* plans, int count, const void* const*
       for (int j = 0; j < i; ++j) require(plans[j] != plans[i], "a plan may appear once in a group");
       require(plans[i]->device == plans[0]->device, "the plans of a group share one device");
       require(batches[i] >= 1 && batches[i] <= hipfmm::kMaxBatch, "batch must be between 1 and 64 moment states");
+      const hipfmm::DevicePlan& d = plans[i]->d;
+      require(moments[i] != nullptr || d.ns == 0, "moments pointer is null");
+      require(fields[i] != nullptr || d.nt == 0, "fields pointer is null");
     }
-    // The recording below ended the process with a host segmentation fault
-    // inside this call on its first run (cause not found); until it is, the
-    // group is refused here rather than recorded.
-    throw std::runtime_error("hipfmm_group_create is switched off in this build");
     DeviceGuard guard(plans[0]->device);
     auto G = std::make_unique<hipfmm_group>();
     G->device = plans[0]->device;
     G->plans.assign(plans, plans + count);
-    // Everything that allocates or waits happens before the capture.
+    G->moments.assign(moments, moments + count);
+    G->fields.assign(fields, fields + count);
+    G->batches.assign(batches, batches + count);
+    // Everything that allocates or waits happens here, not at launch.
     for (int i = 0; i < count; ++i) {
       hipfmm_plan* P = plans[i];
+      if (sorted_self_indices != nullptr) install_identity(P, sorted_self_indices[i]);
       if (P->timings_pending) finalize_timings(P);
       if (P->d.precision == 32)
         ensure_capacity<float>(P, batches[i], false);
       else
         ensure_capacity<double>(P, batches[i], false);
     }
-    HIPFMM_CHECK(hipStreamCreateWithFlags(&G->capture, hipStreamNonBlocking));
-    HIPFMM_CHECK(hipEventCreateWithFlags(&G->done, hipEventDisableTiming));
-    HIPFMM_CHECK(hipStreamBeginCapture(G->capture, hipStreamCaptureModeThreadLocal));
-    try {
-      for (int i = 0; i < count; ++i) {
-        hipfmm_plan* P = plans[i];
-        if (P->d.precision == 32)
-          evaluate_impl<float>(P, moments[i], fields[i], nullptr, batches[i], nullptr, HIPFMM_DEVICE, G->capture,
-                               false, true);
-        else
-          evaluate_impl<double>(P, moments[i], fields[i], nullptr, batches[i], nullptr, HIPFMM_DEVICE, G->capture,
-                                false, true);
-      }
-    } catch (...) {
-      hipGraph_t broken = nullptr;
-      (void)hipStreamEndCapture(G->capture, &broken);
-      if (broken) (void)hipGraphDestroy(broken);
-      (void)hipGetLastError();
-      hipfmm_group_destroy(G.release());
-      throw;
-    }
-    HIPFMM_CHECK(hipStreamEndCapture(G->capture, &G->graph));
-    HIPFMM_CHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+    HIPFMM_CHECK(hipEventCreateWithFlags(&G->fork, hipEventDisableTiming));
     *group = G.release();
   });
 }
@@ -1403,7 +1381,20 @@ int hipfmm_group_launch(hipfmm_group* group, void* stream) {
   return guarded([&] {
     require(group != nullptr, "group is null");
     DeviceGuard guard(group->device);
-    HIPFMM_CHECK(hipGraphLaunch(group->exec, static_cast<hipStream_t>(stream)));
+    const hipStream_t user = static_cast<hipStream_t>(stream);
+    // One fork point for every plan, then one join per plan: the plans' own
+    // streams run concurrently between them.
+    HIPFMM_CHECK(hipEventRecord(group->fork, user));
+    for (size_t i = 0; i < group->plans.size(); ++i) {
+      hipfmm_plan* P = group->plans[i];
+      if (P->d.precision == 32)
+        evaluate_impl<float>(P, group->moments[i], group->fields[i], nullptr, group->batches[i], nullptr,
+                             HIPFMM_DEVICE, user, false, nullptr, group->fork, false);
+      else
+        evaluate_impl<double>(P, group->moments[i], group->fields[i], nullptr, group->batches[i], nullptr,
+                              HIPFMM_DEVICE, user, false, nullptr, group->fork, false);
+    }
+    for (hipfmm_plan* P : group->plans) HIPFMM_CHECK(hipStreamWaitEvent(user, P->ev[kEvL2P], 0));
   });
 }
 
@@ -1412,10 +1403,7 @@ void hipfmm_group_destroy(hipfmm_group* group) {
   int prev = -1;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(group->device);
-  if (group->exec) (void)hipGraphExecDestroy(group->exec);
-  if (group->graph) (void)hipGraphDestroy(group->graph);
-  if (group->done) (void)hipEventDestroy(group->done);
-  if (group->capture) (void)hipStreamDestroy(group->capture);
+  if (group->fork) (void)hipEventDestroy(group->fork);
   if (prev >= 0) (void)hipSetDevice(prev);
   delete group;
 }

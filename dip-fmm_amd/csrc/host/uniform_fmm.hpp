@@ -220,6 +220,10 @@ class UniformFmm {
   // The C-ABI plan, for callers that group several plans into one launch
   // (hipfmm_group_create).
   hipfmm_plan* native_plan() const { return plan_; }
+  // The fixed identity map in sorted order (empty when the plan has none).
+  std::vector<int> fixed_sorted_self_indices() const {
+    return fixed_ids_.has_value() ? sorted_self_indices(fixed_ids_.value()) : std::vector<int>{};
+  }
 
   const UniformTree& tree() const { return tree_; }
   const Basis& basis() const { return basis_; }

@@ -114,8 +114,8 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
       .value("MFMA_BF16X3", HipM2LMode::MfmaBf16x3)
       .value("MFMA_FP16X3", HipM2LMode::MfmaFp16x3);
 
-    // Several plans' device evaluates recorded into one HIP graph
-  // (hipfmm_group_create); launch() replays them all with one launch.
+  // Several plans' device evaluates launched together (hipfmm_group_create);
+  // launch() enqueues them all, the geometries overlapping on the device.
   struct HipGroup {
     hipfmm_group* g = nullptr;
     py::list keep;  // the UniformFmm objects stay alive with the group
@@ -130,14 +130,23 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
              std::vector<hipfmm_plan*> plans;
              std::vector<const void*> mp;
              std::vector<void*> fp;
+             std::vector<std::vector<int>> selfs(n);
+             std::vector<const int*> sp(n, nullptr);
              for (std::size_t i = 0; i < n; ++i) {
-               plans.push_back(py::cast<UniformFmm&>(fmms[i]).native_plan());
+               const UniformFmm& f = py::cast<UniformFmm&>(fmms[i]);
+               plans.push_back(f.native_plan());
                mp.push_back(reinterpret_cast<const void*>(moments[i]));
                fp.push_back(reinterpret_cast<void*>(fields[i]));
+               selfs[i] = f.fixed_sorted_self_indices();
+               if (!selfs[i].empty()) sp[i] = selfs[i].data();
              }
              auto h = std::make_unique<HipGroup>();
-             const int rc = hipfmm_group_create(plans.data(), static_cast<int>(n), mp.data(), fp.data(),
-                                                batches.data(), &h->g);
+             int rc;
+             {
+               py::gil_scoped_release release;
+               rc = hipfmm_group_create(plans.data(), static_cast<int>(n), mp.data(), fp.data(), batches.data(),
+                                        sp.data(), &h->g);
+             }
              if (rc == HIPFMM_ERROR_INVALID_ARGUMENT) throw std::invalid_argument(hipfmm_last_error());
              if (rc != HIPFMM_OK) throw std::runtime_error(hipfmm_last_error());
              h->keep = fmms;
@@ -154,7 +163,8 @@ PYBIND11_MODULE(_cdfmm_hip, m) {
             }
             if (rc != HIPFMM_OK) throw std::runtime_error(hipfmm_last_error());
           },
-          py::arg("stream") = 0, "Replay every plan's evaluate with one graph launch ordered on `stream`.");
+          py::arg("stream") = 0,
+          "Enqueue every plan's evaluate after the work on `stream`; `stream` waits for all of them.");
 
   // Translation-table blobs (host only, no device needed).
   auto tables_dict = [](const OperatorTables& t) {

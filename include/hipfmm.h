@@ -275,17 +275,21 @@ int hipfmm_plan_autotune(hipfmm_plan* plan, int batch);
  * the reference. */
 int hipfmm_plan_set_launch_choices(hipfmm_plan* plan, int m2l_tile_shape, int p2p_kernel_variant);
 
-/* Multi-geometry launch; no counterpart in the reference.  A group records
+/* Multi-geometry launch; no counterpart in the reference.  A group holds
  * the device-memory evaluates of `count` distinct plans (fields only, each
- * plan with its own device moments/fields pointers and batch) into one HIP
- * graph; hipfmm_group_launch replays all of them with one launch on `stream`,
- * the independent geometries' kernels overlapping.  Pointers, batches, M2L
- * modes and launch choices are those at creation; each plan's fixed identity
- * map applies.  A plan belongs to at most one group at a time and must not be
- * evaluated while its group runs. */
+ * plan with its own device moments/fields pointers and batch), validated and
+ * with their state allocated at creation.  hipfmm_group_launch enqueues all
+ * of them after the work already on `stream`: every plan runs on its own
+ * streams from one fork point, so the independent geometries' kernels
+ * overlap on the device, and `stream` waits for all of them.  Pointers,
+ * batches and each plan's identity map (sorted_self_indices: null, or one
+ * entry per plan, each null or as for hipfmm_plan_evaluate, fixed with the
+ * plan as there) are those at creation; M2L modes and launch choices are
+ * read at each launch.  A plan belongs to at
+ * most one group at a time and must not be evaluated while its group runs. */
 typedef struct hipfmm_group hipfmm_group;
 int hipfmm_group_create(hipfmm_plan* const* plans, int count, const void* const* moments, void* const* fields,
-                        const int* batches, hipfmm_group** group);
+                        const int* batches, const int* const* sorted_self_indices, hipfmm_group** group);
 int hipfmm_group_launch(hipfmm_group* group, void* stream);
 void hipfmm_group_destroy(hipfmm_group* group);
 

@@ -1,10 +1,10 @@
 """Launch machinery of the HipFull path (needs one MI355X): the autotuner and
-the multi-geometry group launch recorded as one HIP graph.
+the multi-geometry group launch.
 
 Neither has a counterpart in the reference.  The bar is the path's own: any
-launch choice meets the FP32 bar against the FP64 FMM, and a graph replay
-reproduces the eager evaluate of the same plan bit for bit, on every replay,
-with eager work and synchronisations between the replays.
+launch choice meets the FP32 bar against the FP64 FMM, and a group launch
+reproduces each plan's single evaluate bit for bit, on every launch, with
+other device work and synchronisations between the launches.
 """
 
 import numpy as np
@@ -77,8 +77,9 @@ def test_autotune_on_vector_paths(hip):
     assert st["m2l_tile_shape"] == 0 and st["p2p_kernel_variant"] == 0
 
 
-@pytest.mark.skip(reason="hipfmm_group_create is switched off: its first GPU run ended the process with a host "
-                         "segmentation fault inside the stream-capture recording; cause not found")
+@pytest.mark.skip(reason="the group was first built as one recorded HIP graph; its first GPU run ended the process "
+                         "with a host segmentation fault inside the stream-capture recording (cause not found), so "
+                         "the recording was removed and the group launches eagerly (tested below)")
 def test_group_graph_replays_match_eager(hip):
     import torch
 
@@ -128,16 +129,92 @@ def test_group_graph_replays_match_eager(hip):
     assert rel_l2(outs[2][0].cpu().numpy(), ref.evaluate(m0)["H"]) <= 1e-12
 
 
-def test_group_is_refused_while_switched_off(hip):
+def test_group_launch_matches_single_plans(hip):
     import torch
 
-    rng = np.random.default_rng(971)
-    x = rng.uniform(0, 1, size=(1000, 3))
-    f = make(hip, x, x, 3, 2, 32)
-    m = torch.zeros((4, 1000, 3), device="cuda")
-    h = torch.zeros((4, 1000, 3), device="cuda")
-    with pytest.raises(RuntimeError, match="switched off"):
-        hip.HipGroup([f], [m.data_ptr()], [h.data_ptr()], [4])
+    rng = np.random.default_rng(972)
+    specs = [(3000, 4, 3, 32, 8, None), (20000, 6, 4, 32, 32, "MFMA_BF16X3"), (5000, 4, 3, 64, 3, None),
+             (8000, 8, 3, 32, 32, "MFMA_FP16X3")]
+    fmms, moms, outs, single, pts = [], [], [], [], []
+    stream = torch.cuda.current_stream().cuda_stream
+    for n, p, d, prec, K, mode in specs:
+        x = rng.uniform(0, 1, size=(n, 3))
+        f = make(hip, x, x, p, d, prec, ids=np.arange(n), mode=None if mode is None else getattr(hip.HipM2LMode, mode))
+        dt = torch.float32 if prec == 32 else torch.float64
+        fmms.append(f)
+        pts.append(x)
+        moms.append(torch.tensor(rng.normal(size=(K, n, 3)), dtype=dt, device="cuda"))
+        outs.append(torch.full((K, n, 3), float("nan"), dtype=dt, device="cuda"))
+        single.append(torch.empty((K, n, 3), dtype=dt, device="cuda"))
+    announce("group launch", len(specs))
+    group = hip.HipGroup(fmms, [m.data_ptr() for m in moms], [o.data_ptr() for o in outs],
+                         [m.shape[0] for m in moms])
+    for launch in range(3):
+        if launch == 2:
+            for m in moms:  # the group reads the moments where they live
+                m.mul_(-0.5)
+        for f, m, e in zip(fmms, moms, single):
+            f.evaluate_device(m.data_ptr(), e.data_ptr(), m.shape[0], stream)
+        junk = torch.randn(1 << 20, device="cuda").sum()  # other device work between launches
+        for o in outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
+        group.launch(stream)
+        torch.cuda.synchronize()
+        for i, (o, e) in enumerate(zip(outs, single)):
+            np.testing.assert_array_equal(o.cpu().numpy(), e.cpu().numpy(), err_msg=f"launch {launch} plan {i}")
+        print(f"  launch {launch}: all {len(specs)} geometries bitwise equal to single evaluates "
+              f"({float(junk):.1f})", flush=True)
+    # The FP64 geometry against the oracle, as a value check of the group.
+    n, p, d = specs[2][:3]
+    ref = O.CpuFmm(pts[2], pts[2], p, d, 64, fixed_ids=np.arange(n))
+    m0 = moms[2][0].cpu().numpy()
+    assert rel_l2(outs[2][0].cpu().numpy(), ref.evaluate(m0)["H"]) <= 1e-12
+
+
+def test_group_of_eight_geometries_overlaps(hip):
+    # Eight uniform-cube geometries with N log-uniform in [16k, 1M], p = 6,
+    # the depth giving about 32 points per leaf, K = 32, FP32: the group
+    # against the same evaluates one after another on one stream.
+    import torch
+
+    rng = np.random.default_rng(973)
+    sizes = np.sort(np.exp(rng.uniform(np.log(16384), np.log(1 << 20), size=8)).astype(int))
+    fmms, moms, outs, seq = [], [], [], []
+    for n in sizes:
+        x = rng.uniform(0, 1, size=(int(n), 3))
+        depth = int(np.clip(np.round(np.log(n / 32) / np.log(8)), 2, 5))
+        fmms.append(make(hip, x, x, 6, depth, 32, ids=np.arange(n)))
+        moms.append(torch.randn((32, int(n), 3), device="cuda"))
+        outs.append(torch.empty((32, int(n), 3), device="cuda"))
+        seq.append(torch.empty((32, int(n), 3), device="cuda"))
+    announce("group of eight", " ".join(str(int(n)) for n in sizes))
+    stream = torch.cuda.current_stream().cuda_stream
+    group = hip.HipGroup(fmms, [m.data_ptr() for m in moms], [o.data_ptr() for o in outs], [32] * 8)
+
+    def sequential():
+        for f, m, o in zip(fmms, moms, seq):
+            f.evaluate_device(m.data_ptr(), o.data_ptr(), 32, stream)
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
+            fn()
+        t1.record()
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / reps
+
+    ms_seq = timed(sequential)
+    ms_group = timed(lambda: group.launch(stream))
+    evals = 32 * int(sizes.sum())
+    print(f"  sequential {ms_seq:.2f} ms, group {ms_group:.2f} ms ({evals / (ms_group * 1e-3):.3e} evals/s)",
+          flush=True)
+    for o, e in zip(outs, seq):
+        assert torch.equal(o, e)
+    assert ms_group <= 1.05 * ms_seq
 
 
 def test_group_rejects_duplicates_and_bad_batches(hip):
