@@ -166,6 +166,38 @@ __global__ void __launch_bounds__(kThreads) gather_kernel(const S* __restrict__ 
   }
 }
 
+// The inverse of gather_kernel for the fields: sorted totals Hn[t][3][K] to
+// the user layout out[k][user][3].  A workgroup owns 64 consecutive user
+// points: it reads each point's 3K values contiguously from its sorted slot
+// into LDS, then writes the 64 points' 192 contiguous values of every
+// state.  FP32, N and K multiples of 64 and 4, 16-byte aligned output.
+__global__ void __launch_bounds__(kThreads) scatter_fields_kernel(const float* __restrict__ hs,
+                                                                  const int* __restrict__ inv, int nt, int K,
+                                                                  float* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* tile = reinterpret_cast<float*>(smem_raw);  // [point][3K + 4]
+  const int row = 3 * K + 4, w4 = 3 * K / 4;
+  const int u0 = blockIdx.x * 64;
+  for (int e = threadIdx.x; e < 64 * w4; e += kThreads) {
+    const int u = e / w4, q = e - u * w4;
+    *reinterpret_cast<float4*>(tile + u * row + 4 * q) =
+        reinterpret_cast<const float4*>(hs + int64_t(inv[u0 + u]) * 3 * K)[q];
+  }
+  __syncthreads();
+  constexpr int kRow4 = 3 * 64 / 4;
+  float4* dst = reinterpret_cast<float4*>(out);
+  for (int e = threadIdx.x; e < K * kRow4; e += kThreads) {
+    const int k = e / kRow4, q = e - k * kRow4;
+    float w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 4 * q + i, u = r / 3, c = r - 3 * u;
+      w[i] = tile[u * row + c * K + k];
+    }
+    dst[(int64_t(k) * nt + u0) * 3 / 4 + q] = float4{w[0], w[1], w[2], w[3]};
+  }
+}
+
 // Shared LDS layout of the monomial-based kernels (P2M, L2P).
 struct MonoLayout {
   int NP, KP, JC, TCP;
@@ -795,8 +827,10 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
 // component and 16-state half; the monomial operand is shared by the three
 // components.  One wave per 16-target tile (the near-field tiles), four per
 // workgroup; per wave LDS holds the axis powers and the monomial table
-// mono[g][t].  Writes out[k][user target][3] (and pot[k][user target]).
-template <bool kPot>
+// mono[g][t].  Writes out[k][user target][3] (and pot[k][user target]), or
+// with kSorted the total fields over the near field in sorted order
+// Hn[t][3][K], for scatter_fields_kernel to unsort.
+template <bool kPot, bool kSorted>
 __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float* __restrict__ out,
                                                        float* __restrict__ pot, int K) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -901,11 +935,18 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
     const int tl = 4 * q + r;
     if (tl >= cnt) continue;
     const int ts = t0 + tl;
-    const int user = d.tgt_perm[ts];
+    const int user = kSorted ? 0 : d.tgt_perm[ts];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int k = kc0 + 16 * h + col;
-      if (!kPot || out != nullptr) {
+      if constexpr (kSorted) {
+        // The total overwrites the near field in sorted order; the unsort
+        // follows as its own coalesced pass.
+        float* hn = const_cast<float*>(Hn) + int64_t(ts) * 3 * K + k;
+        hn[0] -= acc[0][h][r];
+        hn[K] -= acc[1][h][r];
+        hn[2 * K] -= acc[2][h][r];
+      } else if (!kPot || out != nullptr) {
         float* o = out + (int64_t(k) * d.nt + user) * 3;
         const float* hn = Hn + int64_t(ts) * 3 * K + k;
         o[0] = hn[0] - acc[0][h][r];
@@ -1198,11 +1239,23 @@ void launch_l2p_unsort(const DevicePlan& d, S* fields, S* potentials, int K, hip
         (size_t(4) * 48 * (d.p + 1) + size_t(4) * NM * 16) * sizeof(float) + size_t(NM + d.nm) * sizeof(int4);
     if (K % 32 == 0 && d.l2p_up != nullptr && d.p2p_tile16_count > 0 && lds <= 65536) {
       const dim3 grid(unsigned((d.p2p_tile16_count + 3) / 4), unsigned(K / 32));
+      // Fields only, whole 64-point tiles: the totals stay sorted and one
+      // coalesced pass unsorts them (a direct unsort writes 12-byte pieces
+      // of 32 rows per target).
+      const bool sorted = !pot && d.nt % 64 == 0 && reinterpret_cast<uintptr_t>(fields) % 16 == 0 &&
+                          size_t(64) * (3 * K + 4) * sizeof(float) <= 65536;
       if (pot)
-        l2p_mfma_kernel<true><<<grid, 256, lds, s>>>(d, fields, potentials, K);
+        l2p_mfma_kernel<true, false><<<grid, 256, lds, s>>>(d, fields, potentials, K);
+      else if (sorted)
+        l2p_mfma_kernel<false, true><<<grid, 256, lds, s>>>(d, fields, nullptr, K);
       else
-        l2p_mfma_kernel<false><<<grid, 256, lds, s>>>(d, fields, nullptr, K);
+        l2p_mfma_kernel<false, false><<<grid, 256, lds, s>>>(d, fields, nullptr, K);
       HIPFMM_CHECK(hipGetLastError());
+      if (sorted) {
+        scatter_fields_kernel<<<unsigned(d.nt / 64), kThreads, size_t(64) * (3 * K + 4) * sizeof(float), s>>>(
+            static_cast<const float*>(d.Hn), d.tgt_inv, d.nt, K, fields);
+        HIPFMM_CHECK(hipGetLastError());
+      }
       return;
     }
   }
