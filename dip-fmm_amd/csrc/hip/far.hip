@@ -778,9 +778,10 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
 #pragma unroll
       for (int u = 0; u < kSteps; ++u) {
         const float* ps = pw + sjv[u] * 3 * PW;
+        // A tile past the last monomial row has row_ok false and multiplies
+        // zeros; its products are never stored.
 #pragma unroll
         for (int j = 0; j < kTW; ++j) {
-          if (wave + 4 * j >= NT) break;
           const float a = (okv[u] && row_ok[j]) ? ps[ex[j].x] * ps[PW + ex[j].y] * ps[2 * PW + ex[j].z] : 0.0f;
 #pragma unroll
           for (int c = 0; c < 3; ++c)
