@@ -203,6 +203,27 @@ def test_batched_matrix_core_paths_match_fp64_fmm(hip, p, depth, n, K, mode):
         assert e <= 1e-5
 
 
+@pytest.mark.parametrize("p,n", [(3, 8192), (5, 8100), (7, 8192), (9, 8192)])
+def test_matrix_core_box_kernels_across_orders(hip, p, n):
+    # FP32, K = 32, depth 3: every far-field box product (P2M, compression,
+    # M2M, L2L, expansion, L2P) takes its matrix-core form where the order
+    # allows (the expansion at p = 9 has 14 row tiles and stays on the vector
+    # ALUs), with N a multiple of 64 (sorted L2P output and a separate unsort)
+    # or not (N = 8100: unsort inside the L2P).
+    rng = np.random.default_rng(500 + p)
+    x = rng.uniform(0, 1, size=(n, 3))
+    ms = rng.normal(size=(32, n, 3))
+    ids = np.arange(n)
+    announce("box kernels across orders", p, n)
+    f = make(hip, x, x, p, 3, 32, ids=ids, mode=hip.HipM2LMode.MFMA_BF16X3)
+    hb = f.evaluate_batch(ms)
+    ref = O.CpuFmm(x, x, p, 3, 64, fixed_ids=ids)
+    for k in (0, 31):
+        e = rel_l2(hb[k], ref.evaluate(ms[k])["H"])
+        print(f"  state {k}: rel L2 vs FP64 FMM {e:.3e}", flush=True)
+        assert e <= 1e-5
+
+
 @pytest.mark.parametrize("p,depth,n,K", [(8, 3, 20000, 1), (8, 4, 30000, 32), (4, 3, 8000, 1)])
 def test_fp16_m2l_is_closer_to_fp64_fmm_than_bf16(hip, p, depth, n, K):
     # FP16 hi/lo operands with per-degree power-of-two scales on the class
