@@ -221,6 +221,11 @@ template <class S>
 void launch_m2l_valu(const DevicePlan& d, int K, hipStream_t s);  // all levels 2..L
 template <class S>
 void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s);
+// Leaf order of `count` points (device arrays; tree.hip): the stable sort by
+// leaf Morton index into permutation[sorted] = input index, and leaf_ranges
+// [2 * leaf] = {begin, end} (count, count for an empty leaf).
+void device_leaf_order(const double* positions, int count, const double* centre, double half_width, int L,
+                       int* permutation, int64_t* leaf_ranges, hipStream_t s);
 // Field gradient dH_c/dx_j: near part into Gn, far part from the leaf locals
 // plus Gn, unsorted to out[k][user target][6] (xx xy xz yy yz zz).
 template <class S>

@@ -1300,6 +1300,43 @@ void hipfmm_plan_destroy(hipfmm_plan* plan) {
   delete plan;
 }
 
+int hipfmm_tree_leaf_order(const double* positions, int count, const double* root_centre, double root_half_width,
+                           int max_level, int memspace, int* permutation, int64_t* leaf_ranges, void* stream) {
+  return guarded([&] {
+    require(count >= 0, "point count must be non-negative");
+    require(root_centre != nullptr && leaf_ranges != nullptr && (count == 0 || (positions && permutation)),
+            "arrays must not be null");
+    require(memspace == HIPFMM_HOST || memspace == HIPFMM_DEVICE, "unknown memory space");
+    if (gfx950_devices() == 0) throw std::runtime_error("no HIP device is available");
+    const int64_t leaves = max_level >= 0 && max_level <= 9 ? int64_t(1) << (3 * max_level) : 0;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (memspace == HIPFMM_DEVICE) {
+      hipfmm::device_leaf_order(positions, count, root_centre, root_half_width, max_level, permutation, leaf_ranges,
+                                s);
+      return;
+    }
+    struct Free {
+      std::vector<void*> ptrs;
+      ~Free() {
+        for (void* p : ptrs) (void)hipFree(p);
+      }
+    } cleanup;
+    double* pos = nullptr;
+    int* perm = nullptr;
+    int64_t* ranges = nullptr;
+    HIPFMM_CHECK(hipMalloc(&pos, std::max<size_t>(size_t(count) * 3 * sizeof(double), 16)));
+    cleanup.ptrs.push_back(pos);
+    HIPFMM_CHECK(hipMalloc(&perm, std::max<size_t>(size_t(count) * sizeof(int), 16)));
+    cleanup.ptrs.push_back(perm);
+    HIPFMM_CHECK(hipMalloc(&ranges, std::max<size_t>(size_t(2 * leaves) * sizeof(int64_t), 16)));
+    cleanup.ptrs.push_back(ranges);
+    if (count) HIPFMM_CHECK(hipMemcpy(pos, positions, size_t(count) * 3 * sizeof(double), hipMemcpyHostToDevice));
+    hipfmm::device_leaf_order(pos, count, root_centre, root_half_width, max_level, perm, ranges, s);
+    if (count) HIPFMM_CHECK(hipMemcpy(permutation, perm, size_t(count) * sizeof(int), hipMemcpyDeviceToHost));
+    HIPFMM_CHECK(hipMemcpy(leaf_ranges, ranges, size_t(2 * leaves) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  });
+}
+
 int hipfmm_direct_field(const double* targets, int nt, const double* sources, int ns, const double* moments,
                         const int* self_indices, double* fields) {
   return guarded([&] {

@@ -95,7 +95,7 @@ UniformTree::UniformTree(const std::vector<Vec3>& sources, const std::vector<Vec
     for (int a = 0; a < 3; ++a) {
       if (!(v[a] >= rmin[a] - kTol && v[a] <= rmax[a] + kTol))
         throw std::invalid_argument("Point lies outside the requested root box");
-      if (L_ == 0) continue;
+      if (L_ == 0 || !(cell > 0.0)) continue;  // a root of zero width: every point in leaf 0
       int i = static_cast<int>(std::floor((v[a] - rmin[a]) / cell));
       idx[a] = std::clamp(i, 0, nb - 1);
     }

@@ -220,6 +220,28 @@ m.def("hip_full_available", &hip_full_available);
     return a;
   });
   m.def("hip_device_description", &hip_device_description);
+  m.def(
+      "device_leaf_order",
+      [](py::object positions, std::array<double, 3> root_centre, double root_half_width, int max_level) {
+        const DoubleArray a = py::cast<DoubleArray>(positions);
+        const auto info = a.request();
+        if (info.ndim != 2 || info.shape[1] != 3) throw std::invalid_argument("positions must have shape (n, 3)");
+        if (max_level < 0 || max_level > 9) throw std::invalid_argument("max_level must be in [0, 9]");
+        const int n = static_cast<int>(info.shape[0]);
+        py::array_t<int> perm(n);
+        py::array_t<int64_t> ranges({py::ssize_t(int64_t(1) << (3 * max_level)), py::ssize_t(2)});
+        int rc;
+        {
+          py::gil_scoped_release release;
+          rc = hipfmm_tree_leaf_order(a.data(), n, root_centre.data(), root_half_width, max_level, HIPFMM_HOST,
+                                      perm.mutable_data(), ranges.mutable_data(), nullptr);
+        }
+        if (rc == HIPFMM_ERROR_INVALID_ARGUMENT) throw std::invalid_argument(hipfmm_last_error());
+        if (rc != HIPFMM_OK) throw std::runtime_error(hipfmm_last_error());
+        return py::make_tuple(perm, ranges);
+      },
+      py::arg("positions"), py::arg("root_centre"), py::arg("root_half_width"), py::arg("max_level"),
+      "Leaf order on the device: (permutation sorted -> input index, leaf ranges (8^L, 2) of [begin, end)).");
 
   py::class_<UniformTreeOptions>(m, "UniformTreeOptions")
       .def(py::init<>())

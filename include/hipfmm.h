@@ -309,6 +309,20 @@ void hipfmm_plan_destroy(hipfmm_plan* plan);
  * form of direct_p2p_reference (src/validation.cpp:71-111; model kernel
  * src/cuda_fmm.cu:518-569).  Host arrays: targets nt x 3, sources and moments
  * ns x 3, self_indices nt (or NULL), fields nt x 3 out. */
+/* Leaf order of the uniform tree computed on the device; replaces the
+ * leaf-key and sort steps of the host tree build (src/uniform_tree.cpp, the
+ * leaf-key lambda and counting sort) with bitwise the same result.
+ *   positions:   count x 3 doubles (memspace: HIPFMM_HOST or HIPFMM_DEVICE;
+ *                permutation and leaf_ranges live in the same space)
+ *   root_centre: 3 doubles (host), root_half_width >= 0 (0: all points
+ *                coincide, all in leaf 0), max_level in [0, 9]
+ *   permutation: count ints, sorted position -> input index (stable)
+ *   leaf_ranges: 2 x 8^max_level int64, {begin, end} of each leaf in Morton
+ *                order, {count, count} for an empty leaf
+ * Returns HIPFMM_ERROR_INVALID_ARGUMENT for a point outside the root box. */
+int hipfmm_tree_leaf_order(const double* positions, int count, const double* root_centre, double root_half_width,
+                           int max_level, int memspace, int* permutation, int64_t* leaf_ranges, void* stream);
+
 int hipfmm_direct_field(const double* targets, int nt, const double* sources, int ns,
                         const double* moments, const int* self_indices, double* fields);
 
