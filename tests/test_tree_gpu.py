@@ -78,3 +78,41 @@ def test_leaf_order_time_at_one_million_points(hip):
     print(f"  1M points, depth 5: host tree build {1e3 * (t1 - t0):.1f} ms, device leaf order "
           f"(with transfers) {1e3 * (t2 - t1):.1f} ms", flush=True)
     np.testing.assert_array_equal(perm, t.source_permutation)
+
+
+def test_leaf_order_through_the_c_abi_on_device_memory(hip):
+    # hipfmm_tree_leaf_order with device operands (torch tensors), ordered on
+    # the caller's stream.
+    import ctypes
+    import os
+
+    import torch
+
+    from conftest import REPO
+
+    lib = ctypes.CDLL(os.path.join(REPO, "dip-fmm_amd", "libhipfmm.so"))
+    lib.hipfmm_tree_leaf_order.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p]
+    lib.hipfmm_last_error.restype = ctypes.c_char_p
+    rng = np.random.default_rng(630)
+    n, depth = 50000, 4
+    x = rng.uniform(-2.0, 3.0, size=(n, 3))
+    t = host_tree(hip, x, depth)
+    c = t.root_centre
+    centre = (ctypes.c_double * 3)(c.x, c.y, c.z)
+    pos = torch.tensor(x, dtype=torch.float64, device="cuda")
+    perm = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    ranges = torch.full((1 << (3 * depth), 2), -1, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    print("case: leaf order through the C-ABI, device memory", flush=True)
+    rc = lib.hipfmm_tree_leaf_order(pos.data_ptr(), n, centre, t.root_half_width, depth, 1, perm.data_ptr(),
+                                    ranges.data_ptr(), ctypes.c_void_p(stream))
+    assert rc == 0, lib.hipfmm_last_error().decode()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(perm.cpu().numpy(), t.source_permutation)
+    leaf0 = ((1 << (3 * depth)) - 1) // 7
+    np.testing.assert_array_equal(ranges.cpu().numpy(), t.node_ranges[leaf0:, 0:2])
+    rc = lib.hipfmm_tree_leaf_order(pos.data_ptr(), n, centre, t.root_half_width, 10, 1, perm.data_ptr(),
+                                    ranges.data_ptr(), ctypes.c_void_p(stream))
+    assert rc == 2 and b"max_level" in lib.hipfmm_last_error()
