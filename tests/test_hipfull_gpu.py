@@ -1,6 +1,6 @@
 """HipFull device path: parity with the reference CpuStatic path (needs one MI355X).
 
-Bars (north star): FP32 plans within rel. L2 1e-5 of CpuStatic in the same
+Bars: FP32 plans within rel. L2 1e-5 of CpuStatic in the same
 precision, FP64 plans within 1e-12; the MFMA split-precision M2L within 1e-5
 of the FP64 FMM at the same (p, depth).  The expected values come from the
 committed golden fixtures written by the compiled reference and from the
