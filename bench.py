@@ -128,6 +128,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-split M2L line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before the steps (the reference kernel "
                          "for scaling PMC byte counters)")
@@ -229,6 +230,25 @@ def main() -> int:
         fmm.set_m2l_mode(hip.HipM2LMode.MFMA_BF16X3)
         log(f"plain VALU K=1: {1e3 * plain_s:.2f} ms/evaluate, {N / plain_s:.3e} evals/s")
 
+    # The same step with the level-scaled FP16 hi/lo M2L (same MFMA rate,
+    # 22 bits per operand instead of 16), reported beside the headline.
+    fp16 = {}
+    fp16_fields = None
+    if not args.no_fp16:
+        fmm.set_m2l_mode(hip.HipM2LMode.MFMA_FP16X3)
+        step()
+        torch.cuda.synchronize()
+        reps = max(1, min(args.steps, 5))
+        t16 = time.perf_counter()
+        for _ in range(reps):
+            step()
+        torch.cuda.synchronize()
+        s16 = (time.perf_counter() - t16) / reps
+        fp16 = {"value": N * K / s16, "ms_per_step": 1e3 * s16}
+        fp16_fields = fields[0].float().cpu().numpy()
+        fmm.set_m2l_mode(hip.HipM2LMode.MFMA_BF16X3)
+        log(f"MFMA FP16x3 K={K}: {1e3 * s16:.2f} ms/step, {N * K / s16:.3e} evals/s")
+
     accuracy = {}
     if not args.no_accuracy:
         del moments, fields
@@ -239,6 +259,8 @@ def main() -> int:
         accuracy["mfma_vs_fp64_fmm"] = rel_l2(result_fields, ref64)
         if plain:
             accuracy["plain_fp32_vs_fp64_fmm"] = rel_l2(plain_fields, ref64)
+        if fp16_fields is not None:
+            accuracy["mfma_fp16x3_vs_fp64_fmm"] = rel_l2(fp16_fields, ref64)
         del f64
         sample = np.random.default_rng(SEED + 7).choice(N, size=4096, replace=False)
         import ctypes
@@ -277,9 +299,10 @@ def main() -> int:
             "n": N, "order": p, "depth": depth, "batch": K, "m2l": "mfma_bf16x3",
         },
         "plain_hip_fp32": plain or None,
+        "mfma_fp16x3": fp16 or None,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "accuracy": accuracy or None,
-        "stages_ms": {k: 1e3 * v for k, v in stage.items() if k != "evaluations"},
+        "stages_ms": {k: 1e3 * v for k, v in stage.items() if k not in ("evaluations", "m2l_kernel_launches")},
         "roofline": roofline,
         "cpu_baseline": baseline,
     }
