@@ -189,3 +189,43 @@ def test_device_field_gradient_matches_host(hip):
     assert e <= 1e-6
     with pytest.raises(ValueError):
         cdfmm_hip.field_gradient(f, torch.zeros((K, n + 1, 3), device="cuda"))
+
+
+@pytest.mark.parametrize("prec,K", [(64, 2), (32, 64)])
+def test_shared_point_gradients_with_the_identity_map(hip, prec, K):
+    # Targets = sources with the identity map: moving a point moves a source
+    # and a target, so dL/dx = dL/dx_t + dL/dx_s; against central differences
+    # of L along a direction that keeps every point in its leaf (FP64), and
+    # against the FP64 gradients (FP32 at K = 64).
+    import torch
+
+    import cdfmm_hip
+
+    rng = np.random.default_rng(1207)
+    n, depth, p = 4000, 3, 6
+    x = rng.uniform(0, 1, size=(n, 3))
+    ms = rng.normal(size=(K, n, 3))
+    gs = rng.normal(size=(K, n, 3))
+    announce("shared-point gradients", prec, K)
+
+    def grads(prec_):
+        f = make(hip, x, x, p, depth, prec_, ids=np.arange(n))
+        d_t, d_s = cdfmm_hip.position_gradients(f, torch.tensor(ms, device="cuda"), torch.tensor(gs, device="cuda"))
+        return (d_t + d_s).cpu().numpy().astype(np.float64)
+
+    total = grads(prec)
+    if prec == 32:
+        e = rel_l2(total, grads(64))
+        print(f"  FP32 vs FP64 shared-point gradient {e:.3e}", flush=True)
+        assert e <= 1e-5
+        return
+
+    def loss(pts):
+        return float(np.sum(gs * make(hip, pts, pts, p, depth, 64, ids=np.arange(n)).evaluate_batch(ms)))
+
+    h = 1e-7
+    delta = rng.normal(size=x.shape) * inside_leaves(x, depth, 1e-3)[:, None]
+    fd = (loss(x + h * delta) - loss(x - h * delta)) / (2 * h)
+    an = float(np.sum(total * delta))
+    print(f"  directional derivative {an:.9e}, differences {fd:.9e}", flush=True)
+    assert abs(an - fd) <= 1e-6 * abs(an)
