@@ -1026,6 +1026,7 @@ void launch_l2p_grad_unsort(const DevicePlan& d, S* gradients, int K, hipStream_
   if (d.Gn == nullptr || d.grad_tab == nullptr) throw invalid_argument("field-gradient buffers are not allocated");
   const int nm2 = d.p >= 2 ? coefficient_count(d.p - 2) : 0;
   const size_t lds = align16(size_t(d.n) * 6 * sizeof(int)) + (size_t(32) * 3 * std::max(d.p - 1, 1) + size_t(32) * std::max(nm2, 1)) * sizeof(S);
+  if (lds > 65536) throw invalid_argument("expansion order too large for the field-gradient kernel's LDS");
   const S scale = S(1.0 / d.potential_scale);
   l2p_grad_kernel<S><<<unsigned(d.leaves), 256, lds, s>>>(d, gradients, K, nm2, scale);
   HIPFMM_CHECK(hipGetLastError());
