@@ -129,6 +129,8 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-split M2L line")
+    ap.add_argument("--no-other-paths", action="store_true",
+                    help="skip the adjoint, potential and field-gradient timings")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before the steps (the reference kernel "
                          "for scaling PMC byte counters)")
@@ -249,6 +251,38 @@ def main() -> int:
         fmm.set_m2l_mode(hip.HipM2LMode.MFMA_BF16X3)
         log(f"MFMA FP16x3 K={K}: {1e3 * s16:.2f} ms/step, {N * K / s16:.3e} evals/s")
 
+    # The other device paths of the same plan at the same workload: the
+    # adjoint (cotangents in, moment gradients out), fields with potentials,
+    # and the field gradient (6 entries per target).
+    other = {}
+    if not args.no_other_paths:
+        reps = max(1, min(args.steps, 3))
+
+        def time_path(run):
+            run()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                run()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t) / reps
+
+        grad = torch.empty_like(moments)
+        sec = time_path(lambda: fmm.evaluate_adjoint_device(fields.data_ptr(), grad.data_ptr(), K, stream))
+        other["adjoint"] = {"ms_per_step": 1e3 * sec, "value": N * K / sec}
+        del grad
+        pot = torch.empty((K, N), dtype=torch.float32, device=dev)
+        sec = time_path(lambda: fmm.evaluate_potential_device(moments.data_ptr(), fields.data_ptr(), pot.data_ptr(),
+                                                              K, stream))
+        other["fields_and_potentials"] = {"ms_per_step": 1e3 * sec, "value": N * K / sec}
+        del pot
+        gmat = torch.empty((K, N, 6), dtype=torch.float32, device=dev)
+        sec = time_path(lambda: fmm.evaluate_field_gradient_device(moments.data_ptr(), gmat.data_ptr(), K, stream))
+        other["field_gradient"] = {"ms_per_step": 1e3 * sec, "value": N * K / sec}
+        del gmat
+        torch.cuda.empty_cache()
+        log("other paths (ms per K-state step): " + ", ".join(f"{k} {v['ms_per_step']:.2f}" for k, v in other.items()))
+
     accuracy = {}
     if not args.no_accuracy:
         del moments, fields
@@ -300,6 +334,7 @@ def main() -> int:
         },
         "plain_hip_fp32": plain or None,
         "mfma_fp16x3": fp16 or None,
+        "other_paths": other or None,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "accuracy": accuracy or None,
         "stages_ms": {k: 1e3 * v for k, v in stage.items() if k not in ("evaluations", "m2l_kernel_launches")},

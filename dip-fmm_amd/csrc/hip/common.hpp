@@ -168,6 +168,7 @@ struct DevicePlan {
   void* Gn = nullptr;          // near-field field gradient Gn[target][6][K] (allocated once asked for)
   void* gradients_out = nullptr;  // host-path staging: K x nt x 6
   int* grad_tab = nullptr;     // [n][6]: index of b - e_c - e_j (xx xy xz yy yz zz), or -1
+  int* grad_up = nullptr;      // [g][8], g of degree <= p - 2: index of g + e_c + e_j (6 used)
   double potential_scale = 1.0;  // frame factor of the potential (root width W for FP32 plans)
   // Harmonic-subspace M2L: L = Q sum_o C_o P M with the level scalings folded
   // into P and Q (nc = 0: the M2L runs on the full n coefficients).

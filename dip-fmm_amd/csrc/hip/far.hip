@@ -1020,11 +1020,119 @@ void launch_p2m(const DevicePlan& d, int K, hipStream_t s) {
   HIPFMM_CHECK(hipGetLastError());
 }
 
+// Far-field gradient on the matrix cores (FP32 plans, K a multiple of 32),
+// laid out like l2p_mfma_kernel: for 16 targets of one leaf and 32 states,
+//   G_e[t][k] = Gn[t][e][k] - sum_g mono_g(t) L[g + e_c + e_j][k]
+// over the monomials g of degree <= p - 2 (the A operand, 4 per step) and
+// the six (c, j) = xx xy xz yy yz zz (a second raising table), as six 16 x 16
+// accumulator tiles per 16-state half; times the frame factor 1/W, written
+// to out[k][user target][6].  One wave per 16-target tile, four per
+// workgroup; four steps' loads are in flight together.
+__global__ void __launch_bounds__(256) l2p_grad_mfma_kernel(const DevicePlan d, float* __restrict__ out, int K,
+                                                           int nm2, float scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = d.n, PW = max(d.p - 1, 1);  // axis powers 0..p-2
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  float* pw = reinterpret_cast<float*>(smem_raw) + wave * (48 * PW);                 // [t][axis][i]
+  float* mono = reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + wave * (nm2 * 16);  // [g][t]
+  int4* ex_l = reinterpret_cast<int4*>(reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + 4 * nm2 * 16);
+  int* up_l = reinterpret_cast<int*>(ex_l + nm2);  // [g][8]
+  for (int i = threadIdx.x; i < nm2; i += 256) ex_l[i] = d.p2m_tab[n + i];
+  for (int i = threadIdx.x; i < nm2 * 8; i += 256) up_l[i] = d.grad_up[i];
+  const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * 4 + wave;
+  const bool valid = item < d.p2p_tile16_count;
+  const int2 tile = valid ? d.p2p_tiles16[item] : int2{0, 0};
+  const int2 trange = valid ? d.leaf_tgt[tile.x] : int2{0, 0};
+  const int t0 = trange.x + tile.y;
+  const int cnt = valid ? min(16, trange.y - t0) : 0;
+  const auto* rel = static_cast<const float4*>(d.tgt_rel);
+  if (lane < 48) {
+    const int t = lane / 3, a = lane % 3;
+    float x = 0.0f;
+    if (t < cnt) {
+      const float4 r = rel[t0 + t];
+      x = a == 0 ? r.x : (a == 1 ? r.y : r.z);
+    }
+    float v = 1.0f;
+    pw[lane * PW] = 1.0f;
+    for (int i = 1; i < PW; ++i) {
+      v = v * x / float(i);
+      pw[lane * PW + i] = v;
+    }
+  }
+  __syncthreads();  // axis powers and tables in LDS
+  for (int w = lane; w < nm2 * 16; w += 64) {
+    const int g = w >> 4, t = w & 15;
+    const int4 e = ex_l[g];
+    const float* pt = pw + t * 3 * PW;
+    mono[w] = pt[e.x] * pt[PW + e.y] * pt[2 * PW + e.z];
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int col = lane & 15, q = lane >> 4;
+  const int kc0 = int(blockIdx.y) * 32;
+  const int64_t node = level_offset(d.L) + tile.x;
+  const float* Lb = static_cast<const float*>(d.Lc) + node * int64_t(n) * K + kc0 + col;
+  f32x4 acc[6][2];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) acc[e][0] = acc[e][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  constexpr int kSteps = 4;
+  for (int g00 = 0; g00 < nm2; g00 += 4 * kSteps) {
+    float av[kSteps], bv[kSteps][6][2];
+    bool okv[kSteps];
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) {
+      const int g = g00 + 4 * u + q;
+      okv[u] = g < nm2;
+      const int gl = okv[u] ? g : 0;
+      av[u] = okv[u] ? mono[gl * 16 + col] : 0.0f;
+#pragma unroll
+      for (int e = 0; e < 6; ++e)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) bv[u][e][h] = Lb[int64_t(up_l[gl * 8 + e]) * K + 16 * h];
+    }
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u)
+#pragma unroll
+      for (int e = 0; e < 6; ++e)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          acc[e][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], okv[u] ? bv[u][e][h] : 0.0f, acc[e][h], 0, 0, 0);
+  }
+  // acc[e][h][r]: target t0 + 4q + r, state kc0 + 16h + col.
+  const float* Gn = static_cast<const float*>(d.Gn);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int tl = 4 * q + r;
+    if (tl >= cnt) continue;
+    const int ts = t0 + tl;
+    const int user = d.tgt_perm[ts];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = kc0 + 16 * h + col;
+      float* o = out + (int64_t(k) * d.nt + user) * 6;
+      const float* gn = Gn + int64_t(ts) * 6 * K + k;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) o[e] = scale * (gn[int64_t(e) * K] - acc[e][h][r]);
+    }
+  }
+}
+
 template <class S>
 void launch_l2p_grad_unsort(const DevicePlan& d, S* gradients, int K, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
   if (d.Gn == nullptr || d.grad_tab == nullptr) throw invalid_argument("field-gradient buffers are not allocated");
   const int nm2 = d.p >= 2 ? coefficient_count(d.p - 2) : 0;
+  if constexpr (sizeof(S) == 4) {
+    const size_t lds_m = (size_t(4) * 48 * std::max(d.p - 1, 1) + size_t(4) * nm2 * 16) * sizeof(float) +
+                         size_t(nm2) * sizeof(int4) + size_t(nm2) * 8 * sizeof(int);
+    if (K % 32 == 0 && nm2 > 0 && d.grad_up != nullptr && d.p2p_tile16_count > 0 && lds_m <= 65536) {
+      const dim3 grid(unsigned((d.p2p_tile16_count + 3) / 4), unsigned(K / 32));
+      l2p_grad_mfma_kernel<<<grid, 256, lds_m, s>>>(d, gradients, K, nm2, float(1.0 / d.potential_scale));
+      HIPFMM_CHECK(hipGetLastError());
+      return;
+    }
+  }
   const size_t lds = align16(size_t(d.n) * 6 * sizeof(int)) + (size_t(32) * 3 * std::max(d.p - 1, 1) + size_t(32) * std::max(nm2, 1)) * sizeof(S);
   if (lds > 65536) throw invalid_argument("expansion order too large for the field-gradient kernel's LDS");
   const S scale = S(1.0 / d.potential_scale);

@@ -667,10 +667,109 @@ void launch_p2p_grad_t(const DevicePlan& d, int K, hipStream_t s) {
 
 }  // namespace
 
+// Near-field gradient on the matrix cores (FP32 plans, K a multiple of 32),
+// targets as rows like p2p_mfma16_kernel: with g = 3/(4 pi |r|^4) the
+// derivative of the pair tensor is the fully symmetric 3-tensor
+//   W_cjd = g (delta_cj u_d + delta_jd u_c + delta_cd u_j - 5 u_c u_j u_d),
+// 10 distinct entries, and G_cj += sum_d W_cjd m_d: 18 products per 4
+// sources and 16 states, six 16 x 16 accumulator tiles per column half.
+// Padding slots and the identity pair get |r|^-1 = 0.  Writes Gn[t][6][K]
+// (xx xy xz yy yz zz) for the far-field gradient kernel to complete.
+__global__ void __launch_bounds__(64 * kP2PWaves) p2p_grad_mfma16_kernel(const DevicePlan d,
+                                                                        const int2* __restrict__ tiles,
+                                                                        int tile_count, int K) {
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * kP2PWaves + wave;
+  if (item >= tile_count) return;
+  const int2 tile = tiles[item];
+  const int leaf = tile.x;
+  const int2 trange = d.leaf_tgt[leaf];
+  const int col = lane & 15, q = lane >> 4;
+  const bool row_ok = trange.x + tile.y + col < trange.y;
+  const int t = row_ok ? trange.x + tile.y + col : trange.x;
+  const PairPos<float> tp = PairPos<float>::load(d.tgt_pos, d.nt, t);
+  const int self = row_ok ? d.self[t] : -2;
+  const int kc0 = int(blockIdx.y) * 32;
+  const float* sm = static_cast<const float*>(d.sm);
+  f32x4 acc[6][2];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) acc[e][0] = acc[e][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int nb = 1 << d.L;
+  int lx, ly, lz;
+  demorton3(uint32_t(leaf), d.L, lx, ly, lz);
+  const float c34pi = 0.23873241463784300365332564505877154f;  // 3/(4 pi)
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = lx + dx, y = ly + dy, z = lz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+        const int2 sr = d.leaf_src[morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L)];
+        if (sr.x == sr.y) continue;
+        const P2PSources src{static_cast<const float4*>(d.src_pos) + sr.x,
+                             static_cast<const float4*>(d.src_pos) + d.ns + sr.x, sm + int64_t(sr.x) * 3 * K + kc0};
+        for (int j00 = sr.x; j00 < sr.y; j00 += 8) {
+          P2PRaw raw[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) raw[u] = p2p_load(src, j00 + 4 * u + q, sr.x, sr.y, K, col);
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            float rx, ry, rz;
+            tp.diff(raw[u].sp, rx, ry, rz);
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            const bool use = row_ok & (raw[u].s < sr.y) & (raw[u].s != self);
+            // A coincident pair that is used gives inf, hence NaN entries.
+            const float rinv = use ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+            const float g = c34pi * rinv * rinv * rinv * rinv;
+            const float ux = rx * rinv, uy = ry * rinv, uz = rz * rinv;
+            const float g5 = 5.0f * g;
+            // W_cjd with sorted indices 000 001 002 011 012 022 111 112 122 222.
+            const float w000 = g * ux * (3.0f - 5.0f * ux * ux);
+            const float w001 = uy * (g - g5 * ux * ux);
+            const float w002 = uz * (g - g5 * ux * ux);
+            const float w011 = ux * (g - g5 * uy * uy);
+            const float w012 = -g5 * ux * uy * uz;
+            const float w022 = ux * (g - g5 * uz * uz);
+            const float w111 = g * uy * (3.0f - 5.0f * uy * uy);
+            const float w112 = uz * (g - g5 * uy * uy);
+            const float w122 = uy * (g - g5 * uz * uz);
+            const float w222 = g * uz * (3.0f - 5.0f * uz * uz);
+            const float W[6][3] = {{w000, w001, w002}, {w001, w011, w012}, {w002, w012, w022},
+                                   {w011, w111, w112}, {w012, w112, w122}, {w022, w122, w222}};
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+              for (int dd = 0; dd < 3; ++dd)
+#pragma unroll
+                for (int e = 0; e < 6; ++e)
+                  acc[e][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(W[e][dd], raw[u].b[dd][h], acc[e][h], 0, 0, 0);
+          }
+        }
+      }
+  // acc[e][h][i]: target tile.y + 4q + i, state kc0 + 16h + col.
+  float* Gn = static_cast<float*>(d.Gn);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t ot = int64_t(trange.x) + tile.y + 4 * q + i;
+    if (ot >= trange.y) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int e = 0; e < 6; ++e) Gn[(ot * 6 + e) * K + kc0 + 16 * h + col] = acc[e][h][i];
+  }
+}
+
 template <class S>
 void launch_p2p_grad(const DevicePlan& d, int K, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
   if (d.Gn == nullptr) throw invalid_argument("near-field gradient buffer is not allocated");
+  if constexpr (sizeof(S) == 4) {
+    if (K % 32 == 0 && d.p2p_tile16_count > 0) {
+      const dim3 grid(unsigned((d.p2p_tile16_count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
+      p2p_grad_mfma16_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, d.p2p_tile16_count, K);
+      HIPFMM_CHECK(hipGetLastError());
+      return;
+    }
+  }
   if (K <= 8)
     launch_p2p_grad_t<S, 1>(d, K, s);
   else if (K <= 32)

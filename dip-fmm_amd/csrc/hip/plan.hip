@@ -415,6 +415,29 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
     }
     d.l2p_up = upload(P, up.data(), up.size());
   }
+  // Raising table of the matrix-core field gradient: g (degree <= p - 2) ->
+  // g + e_c + e_j for (c, j) = xx xy xz yy yz zz, as two int4 per g.
+  if (p >= 2) {
+    const int nm2 = hipfmm::coefficient_count(p - 2);
+    const int cj[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    std::vector<int> up2(size_t(nm2) * 8, 0);
+    for (int g = 0; g < nm2; ++g) {
+      const int4 e = tab[n + g];
+      for (int q = 0; q < 6; ++q) {
+        int x[3] = {e.x, e.y, e.z};
+        ++x[cj[q][0]];
+        ++x[cj[q][1]];
+        int idx = -1;
+        for (int b = 0; b < n; ++b) {
+          const int4 f = tab[n + b];
+          if (f.x == x[0] && f.y == x[1] && f.z == x[2]) idx = b;
+        }
+        if (idx < 0) throw std::logic_error("basis second raising table");
+        up2[size_t(g) * 8 + q] = idx;
+      }
+    }
+    d.grad_up = upload(P, up2.data(), up2.size());
+  }
   // M2L class matrices keyed by offset slot.
   std::vector<int> slot(hipfmm::kOffsetSlots, -1), slot_of_class(std::max(desc.m2l_class_count, 1), -1);
   for (int c = 0; c < desc.m2l_class_count; ++c) {
