@@ -198,6 +198,30 @@ __global__ void __launch_bounds__(kThreads) scatter_fields_kernel(const float* _
   }
 }
 
+// The same for the potentials: sorted Pn[t][K] to pot[k][user].  A
+// workgroup owns 64 consecutive user points: each point's K values are one
+// contiguous read, and each state's 64 values one contiguous write.
+__global__ void __launch_bounds__(kThreads) scatter_potentials_kernel(const float* __restrict__ ps,
+                                                                      const int* __restrict__ inv, int nt, int K,
+                                                                      float* __restrict__ pot) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* tile = reinterpret_cast<float*>(smem_raw);  // [point][K + 4]
+  const int row = K + 4, k4 = K / 4;
+  const int u0 = blockIdx.x * 64;
+  for (int e = threadIdx.x; e < 64 * k4; e += kThreads) {
+    const int u = e / k4, q = e - u * k4;
+    *reinterpret_cast<float4*>(tile + u * row + 4 * q) =
+        reinterpret_cast<const float4*>(ps + int64_t(inv[u0 + u]) * K)[q];
+  }
+  __syncthreads();
+  float4* dst = reinterpret_cast<float4*>(pot);
+  for (int e = threadIdx.x; e < K * 16; e += kThreads) {
+    const int k = e >> 4, q = e & 15;
+    dst[(int64_t(k) * nt + u0) / 4 + q] = float4{tile[(4 * q) * row + k], tile[(4 * q + 1) * row + k],
+                                                 tile[(4 * q + 2) * row + k], tile[(4 * q + 3) * row + k]};
+  }
+}
+
 // Shared LDS layout of the monomial-based kernels (P2M, L2P).
 struct MonoLayout {
   int NP, KP, JC, TCP;
@@ -943,10 +967,12 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
       if constexpr (kSorted) {
         // The total overwrites the near field in sorted order; the unsort
         // follows as its own coalesced pass.
-        float* hn = const_cast<float*>(Hn) + int64_t(ts) * 3 * K + k;
-        hn[0] -= acc[0][h][r];
-        hn[K] -= acc[1][h][r];
-        hn[2 * K] -= acc[2][h][r];
+        if (!kPot || out != nullptr) {
+          float* hn = const_cast<float*>(Hn) + int64_t(ts) * 3 * K + k;
+          hn[0] -= acc[0][h][r];
+          hn[K] -= acc[1][h][r];
+          hn[2 * K] -= acc[2][h][r];
+        }
       } else if (!kPot || out != nullptr) {
         float* o = out + (int64_t(k) * d.nt + user) * 3;
         const float* hn = Hn + int64_t(ts) * 3 * K + k;
@@ -955,8 +981,12 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
         o[2] = hn[2 * K] - acc[2][h][r];
       }
       if constexpr (kPot) {
-        const float pn = static_cast<const float*>(d.Pn)[int64_t(ts) * K + k];
-        pot[int64_t(k) * d.nt + user] = float(d.potential_scale) * (pn + accp[h][r]);
+        float* pn = static_cast<float*>(d.Pn) + int64_t(ts) * K + k;
+        const float phi = float(d.potential_scale) * (*pn + accp[h][r]);
+        if constexpr (kSorted)
+          *pn = phi;  // unsorted by scatter_potentials_kernel
+        else
+          pot[int64_t(k) * d.nt + user] = phi;
       }
     }
   }
@@ -1352,18 +1382,26 @@ void launch_l2p_unsort(const DevicePlan& d, S* fields, S* potentials, int K, hip
       // Fields only, whole 64-point tiles: the totals stay sorted and one
       // coalesced pass unsorts them (a direct unsort writes 12-byte pieces
       // of 32 rows per target).
-      const bool sorted = !pot && d.nt % 64 == 0 && reinterpret_cast<uintptr_t>(fields) % 16 == 0 &&
+      const bool sorted = d.nt % 64 == 0 && reinterpret_cast<uintptr_t>(fields) % 16 == 0 &&
+                          reinterpret_cast<uintptr_t>(potentials) % 16 == 0 &&
                           size_t(64) * (3 * K + 4) * sizeof(float) <= 65536;
-      if (pot)
+      if (pot && sorted)
+        l2p_mfma_kernel<true, true><<<grid, 256, lds, s>>>(d, fields, potentials, K);
+      else if (pot)
         l2p_mfma_kernel<true, false><<<grid, 256, lds, s>>>(d, fields, potentials, K);
       else if (sorted)
         l2p_mfma_kernel<false, true><<<grid, 256, lds, s>>>(d, fields, nullptr, K);
       else
         l2p_mfma_kernel<false, false><<<grid, 256, lds, s>>>(d, fields, nullptr, K);
       HIPFMM_CHECK(hipGetLastError());
-      if (sorted) {
+      if (sorted && fields != nullptr) {
         scatter_fields_kernel<<<unsigned(d.nt / 64), kThreads, size_t(64) * (3 * K + 4) * sizeof(float), s>>>(
             static_cast<const float*>(d.Hn), d.tgt_inv, d.nt, K, fields);
+        HIPFMM_CHECK(hipGetLastError());
+      }
+      if (sorted && pot) {
+        scatter_potentials_kernel<<<unsigned(d.nt / 64), kThreads, size_t(64) * (K + 4) * sizeof(float), s>>>(
+            static_cast<const float*>(d.Pn), d.tgt_inv, d.nt, K, potentials);
         HIPFMM_CHECK(hipGetLastError());
       }
       return;

@@ -85,15 +85,17 @@ def test_coincident_pair_potential_is_nan(hip):
         assert rel_l2(phi[2:], g[f"coincident/phi{prec}"][2:]) <= BARS[prec]
 
 
-@pytest.mark.parametrize("mode", ["VALU", "MFMA_BF16X3"])
-def test_batched_potentials_on_matrix_cores(hip, mode):
+@pytest.mark.parametrize("mode,n", [("VALU", 12000), ("MFMA_BF16X3", 12000), ("MFMA_BF16X3", 8192)])
+def test_batched_potentials_on_matrix_cores(hip, mode, n):
     # K = 32 on an FP32 plan: the near-field potential is the fourth row per
-    # target of the matrix-core P2P, the far potential a fourth L2P row.
+    # target of the matrix-core P2P, the far potential a fourth L2P row.  At
+    # N = 8192 (a multiple of 64) the totals stay sorted and are unsorted by
+    # their own coalesced passes.
     rng = np.random.default_rng(900)
-    n, K = 12000, 32
+    K = 32
     x = rng.uniform(0, 1, size=(n, 3))
     ms = rng.normal(size=(K, n, 3))
-    announce("batched potentials", mode)
+    announce("batched potentials", mode, n)
     f = make(hip, x, x, 6, 3, 32, ids=np.arange(n), mode=getattr(hip.HipM2LMode, mode))
     out = f.evaluate_batch_outputs(ms, output="both")
     hb = f.evaluate_batch(ms)
@@ -110,14 +112,15 @@ def test_batched_potentials_on_matrix_cores(hip, mode):
     assert rel_l2(only["phi"], out["phi"]) <= 1e-6
 
 
-def test_device_resident_potentials(hip):
+@pytest.mark.parametrize("n", [6000, 8192])
+def test_device_resident_potentials(hip, n):
     import torch
 
     rng = np.random.default_rng(910)
-    n, K = 6000, 32
+    K = 32
     x = rng.uniform(0, 1, size=(n, 3))
     ms = rng.normal(size=(K, n, 3))
-    announce("device potentials")
+    announce("device potentials", n)
     f = make(hip, x, x, 4, 3, 32, ids=np.arange(n), mode=hip.HipM2LMode.MFMA_BF16X3)
     m_dev = torch.tensor(ms, dtype=torch.float32, device="cuda")
     phi = torch.empty((K, n), dtype=torch.float32, device="cuda")
