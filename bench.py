@@ -129,6 +129,8 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-split M2L line")
+    ap.add_argument("--near-variant", type=int, default=0,
+                    help="pin the near-field kernel (codes of hip_plan_statistics; 0 = built-in default)")
     ap.add_argument("--no-other-paths", action="store_true",
                     help="skip the adjoint, potential and field-gradient timings")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -153,6 +155,8 @@ def main() -> int:
     t0 = time.perf_counter()
     x = uniform_geometry(N, SEED)
     fmm = make_plan(hip, x, p, depth, hip.StaticPrecision.FLOAT32, hip.HipM2LMode.MFMA_BF16X3)
+    if args.near_variant:
+        fmm.set_launch_choices(0, args.near_variant)
     log(f"plan setup: {time.perf_counter() - t0:.1f}s")
     stats = fmm.hip_plan_statistics
     moments = torch.from_numpy(moment_states(N, K, SEED)).to(dev)

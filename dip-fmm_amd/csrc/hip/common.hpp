@@ -135,6 +135,14 @@ struct DevicePlan {
   int p2p_tile4_count = 0;
   int2* p2p_tiles16 = nullptr; // (leaf, first target) of each 16-target tile (targets-as-rows kernel)
   int p2p_tile16_count = 0;
+  // Moments of the bf16 three-piece near field, in blocks of 8 sources of
+  // one leaf: leaf l's sources fill blocks [blk8_start[l], blk8_start[l+1]);
+  // blk8_src[b] = (first source, sources in the block).
+  int* blk8_start = nullptr;
+  int2* blk8_src = nullptr;
+  int blk8_count = 0;
+  void* smb = nullptr;         // bf16 [block][3 components][3 pieces][K][8 sources]
+  int smb_blocks = 0;          // blocks smb holds (both roles)
   // basis tables
   int4* p2m_tab = nullptr;  // per alpha: index(alpha-e_c) or -1 for c=0..2, w = sign (+1/-1)
   int4* l2p_up = nullptr;   // per g of degree < p: index(g + e_c) for c = 0..2
@@ -202,7 +210,8 @@ struct DevicePlan {
   // Launch choices (set by the autotuner; 0 = built-in default).
   int m2l_shape = 0;    // MFMA M2L tile shape code (kM2LShapes)
   int p2p_variant = 0;  // near field at K % 32 == 0 (FP32): 1 vector ALU, 2 matrix cores with
-                        // (target, component) rows, 3 (= default) matrix cores with target rows
+                        // (target, component) rows, 3 (= default) matrix cores with target rows,
+                        // 4 matrix cores on bf16 three-piece operands (fields only)
 };
 
 // MFMA M2L tile shapes of the harmonic form (<= 3 row tiles): the code is

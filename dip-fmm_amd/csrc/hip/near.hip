@@ -517,11 +517,228 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_mfma16_kernel(const Device
   }
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// (a, b) rounded to nearest bf16 and packed, a in the low half
+// (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(floatx2{a, b}, bf16x2));
+}
+__device__ __forceinline__ float bf16_low(uint32_t p) { return __builtin_bit_cast(float, p << 16); }
+__device__ __forceinline__ float bf16_high(uint32_t p) { return __builtin_bit_cast(float, p & 0xffff0000u); }
+
+// Three-piece bf16 split of 8 values, x = hi + mid + lo with each piece
+// rounded to nearest: the pieces hold 24 significant bits of x, as FP32 does.
+// Element e of a piece sits in half e % 2 of its word e / 2.  Infinities and
+// NaNs give NaN pieces.
+__device__ __forceinline__ void split3_bf16(const float (&x)[8], uint4& hi, uint4& mid, uint4& lo) {
+  uint32_t h[4], m[4], l[4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const float x0 = x[2 * w], x1 = x[2 * w + 1];
+    h[w] = pack_bf16(x0, x1);
+    const float r0 = x0 - bf16_low(h[w]), r1 = x1 - bf16_high(h[w]);
+    m[w] = pack_bf16(r0, r1);
+    l[w] = pack_bf16(r0 - bf16_low(m[w]), r1 - bf16_high(m[w]));
+  }
+  hi = uint4{h[0], h[1], h[2], h[3]};
+  mid = uint4{m[0], m[1], m[2], m[3]};
+  lo = uint4{l[0], l[1], l[2], l[3]};
+}
+
+// The sorted moments as B operands of p2p_bf16_kernel: per block of 8
+// sources of one leaf, smb[block][c'][piece][k][8] (pieces hi, mid, lo of
+// m[s][c'][k]), zero past the leaf's last source.  Thread = (block, state);
+// reads are coalesced over the states, writes are 16-byte rows.
+__global__ void __launch_bounds__(256) split_moments_kernel(const DevicePlan d, int K) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(d.blk8_count) * K) return;
+  const int b = int(i / K), k = int(i % K);
+  const int2 bs = d.blk8_src[b];
+  const float* sm = static_cast<const float*>(d.sm);
+  uint4* out = static_cast<uint4*>(d.smb);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float x[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = e < bs.y ? sm[(int64_t(bs.x + e) * 3 + c) * K + k] : 0.0f;
+    uint4 h, m, l;
+    split3_bf16(x, h, m, l);
+    const int64_t at = (int64_t(b) * 3 + c) * 3 * K + k;
+    out[at] = h;
+    out[at + K] = m;
+    out[at + 2 * K] = l;
+  }
+}
+
+__device__ __forceinline__ f32x4 mfma_bf16(const uint4& a, const uint4& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+
+// part[h] += D_ab m_b for both 16-state halves from the pieces of D_ab (A
+// operands) and of m_b (B operands): the six products of pieces whose orders
+// sum to at most 2, smallest first.
+__device__ __forceinline__ void p2p_bf16_products(f32x4 (&part)[2], const uint4& ah, const uint4& am, const uint4& al,
+                                                  const uint4 (&b)[3][2]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    part[h] = mfma_bf16(am, b[1][h], part[h]);
+    part[h] = mfma_bf16(al, b[0][h], part[h]);
+    part[h] = mfma_bf16(ah, b[2][h], part[h]);
+    part[h] = mfma_bf16(am, b[0][h], part[h]);
+    part[h] = mfma_bf16(ah, b[1][h], part[h]);
+    part[h] = mfma_bf16(ah, b[0][h], part[h]);
+  }
+}
+
+// List-1 P2P on the matrix cores with bf16 operands
+// (v_mfma_f32_16x16x32_bf16), for FP32 plans with K a multiple of 32, fields
+// only.  Both operands are split in three bf16 pieces (hi + mid + lo, 24
+// significant bits) and the six piece products whose orders sum to at most 2
+// are kept, D m ~ Dh mh + Dh mm + Dm mh + Dh ml + Dl mh + Dm mm: the dropped
+// ones are below 2^-26 |D m|, under the rounding of FP32 operands.  Against
+// the FP32 matrix-core product (16x16x4, 32 cycles for 2048 flops) the six
+// bf16 products issue 2.7x fewer matrix-core cycles per pair.
+//
+// One wave owns 16 targets of one leaf and 32 states; the depth of a product
+// is 32 sources of one neighbour leaf, and lane (row i, group g) forms the
+// pairs of target i with that block's sources 8g..8g+7: their offsets r, then
+// the 6 distinct entries D_ab = 3 f r_a r_b / |r|^2 - delta_ab f with
+// f = |r|^-3 / (4 pi), each split into pieces for the products of both
+// components it couples.  Lane layout (CDNA4 16x16x32 bf16): A[i][k] in lane
+// i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8;
+// D[i][j] in lane j + 16 (i / 4), register i % 4.  The moments' pieces come
+// from split_moments_kernel in 8-source blocks, so a lane's B operand is one
+// 16-byte load; a group past the leaf's last source reads the leaf's first
+// block, and its pairs, padding pairs and the identity pair have |r|^-1 = 0
+// and so zero entries.  Coincident pairs give NaN.  Blocked summation per 32
+// sources as in the FP32 kernels.
+__global__ void __launch_bounds__(64 * kP2PWaves) p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
+                                                                   int tile_count, int K) {
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * kP2PWaves + wave;
+  if (item >= tile_count) return;
+  const int2 tile = tiles[item];
+  const int leaf = tile.x;
+  const int2 trange = d.leaf_tgt[leaf];
+  const int col = lane & 15, g = lane >> 4;  // A row / B and D column; source group
+  const bool row_ok = trange.x + tile.y + col < trange.y;
+  const int t = row_ok ? trange.x + tile.y + col : trange.x;
+  const PairPos<float> tp = PairPos<float>::load(d.tgt_pos, d.nt, t);
+  const int self = row_ok ? d.self[t] : -2;
+  const int kc0 = int(blockIdx.y) * 32;
+  const uint4* mb = static_cast<const uint4*>(d.smb);
+  f32x4 acc[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) acc[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int nb = 1 << d.L;
+  int lx, ly, lz;
+  demorton3(uint32_t(leaf), d.L, lx, ly, lz);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = lx + dx, y = ly + dy, z = lz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
+        const uint32_t nl = morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L);
+        const int2 sr = d.leaf_src[nl];
+        const int n = sr.y - sr.x;
+        const int pb = d.blk8_start[nl];
+        const float4* shi = static_cast<const float4*>(d.src_pos) + sr.x;
+        const float4* slo = shi + d.ns;
+        for (int jb = 0; jb < n; jb += 32) {
+          const int o0 = jb + 8 * g;
+          const int blk = pb + (o0 < n ? (o0 >> 3) : 0);
+          uint4 b[3][3][2];  // [component][piece][state half]
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int pc = 0; pc < 3; ++pc)
+#pragma unroll
+              for (int h = 0; h < 2; ++h) b[c][pc][h] = mb[((int64_t(blk) * 3 + c) * 3 + pc) * K + kc0 + 16 * h + col];
+          float r[3][8], f[8], gf[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int o = o0 + e;
+            const uint32_t j = o < n ? uint32_t(o) : 0u;
+            const PairPos<float> sp{shi[j], slo[j]};
+            tp.diff(sp, r[0][e], r[1][e], r[2][e]);
+            const float r2 = r[0][e] * r[0][e] + r[1][e] * r[1][e] + r[2][e] * r[2][e];
+            const bool use = row_ok & (o < n) & (sr.x + o != self);
+            const float rinv = use ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+            const float ri2 = rinv * rinv;
+            f[e] = ri2 * rinv * 0.079577471545947667884441881686257181f;
+            gf[e] = 3.0f * f[e] * ri2;
+          }
+          f32x4 part[3][2];
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int bc = a; bc < 3; ++bc) {
+              float v[8];
+#pragma unroll
+              for (int e = 0; e < 8; ++e) v[e] = gf[e] * r[a][e] * r[bc][e] - (a == bc ? f[e] : 0.0f);
+              uint4 ph, pm, pl;
+              split3_bf16(v, ph, pm, pl);
+              const uint4 bb[3][2] = {{b[bc][0][0], b[bc][0][1]}, {b[bc][1][0], b[bc][1][1]},
+                                      {b[bc][2][0], b[bc][2][1]}};
+              p2p_bf16_products(part[a], ph, pm, pl, bb);
+              if (a != bc) {
+                const uint4 ba[3][2] = {{b[a][0][0], b[a][0][1]}, {b[a][1][0], b[a][1][1]}, {b[a][2][0], b[a][2][1]}};
+                p2p_bf16_products(part[bc], ph, pm, pl, ba);
+              }
+            }
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) acc[c][h] += part[c][h];
+        }
+      }
+  // acc[c][h][i]: target tile.y + 4g + i, state kc0 + 16h + col.
+  float* Hn = static_cast<float*>(d.Hn);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t ot = int64_t(trange.x) + tile.y + 4 * g + i;
+    if (ot >= trange.y) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = kc0 + 16 * h + col;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Hn[(ot * 3 + c) * K + k] = acc[c][h][i];
+    }
+  }
+}
+
+void launch_p2p_bf16(const DevicePlan& d, int K, hipStream_t s) {
+  if (d.smb == nullptr || d.smb_blocks < d.blk8_count)
+    throw invalid_argument("bf16 near-field moment buffer is not allocated");
+  const int count = d.p2p_tile16_count;
+  if (count == 0 || d.blk8_count == 0) return;
+  const int64_t threads = int64_t(d.blk8_count) * K;
+  split_moments_kernel<<<unsigned((threads + 255) / 256), 256, 0, s>>>(d, K);
+  HIPFMM_CHECK(hipGetLastError());
+  const dim3 grid(unsigned((count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
+  p2p_bf16_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, count, K);
+  HIPFMM_CHECK(hipGetLastError());
+}
+
 template <class S>
 void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
   if (potential && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
   if constexpr (sizeof(S) == 4) {
+    if (K % 32 == 0 && d.p2p_variant == 4 && !potential) {
+      launch_p2p_bf16(d, K, s);
+      return;
+    }
     if (K % 32 == 0 && d.p2p_variant != 1 && d.p2p_variant != 2) {
       const int count = d.p2p_tile16_count;
       if (count == 0) return;

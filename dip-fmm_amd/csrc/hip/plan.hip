@@ -67,6 +67,9 @@ struct hipfmm_plan {
   int adj_tile_count = 0;
   int2* adj_tiles16 = nullptr;
   int adj_tile16_count = 0;
+  int* adj_blk8_start = nullptr;
+  int2* adj_blk8_src = nullptr;
+  int adj_blk8_count = 0;
   int* adj_self = nullptr;
   bool adj_self_valid = true;  // false when two targets share one source
   bool potential_used = false;  // keep the near-field potential buffers in the state
@@ -353,6 +356,23 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   };
   d.p2p_tiles16 = tiles16_of(lt, d.p2p_tile16_count);
   P->adj_tiles16 = tiles16_of(ls, P->adj_tile16_count);
+  // Source blocks of 8 of the bf16 three-piece near field, per source leaf
+  // (the adjoint's source leaves are the target leaves).
+  auto blocks8_of = [&](const std::vector<int2>& ranges8, int*& start, int2*& src, int& count) {
+    std::vector<int> st(d.leaves + 1);
+    std::vector<int2> bs;
+    for (int64_t leaf = 0; leaf < d.leaves; ++leaf) {
+      st[leaf] = int(bs.size());
+      for (int s0 = ranges8[leaf].x; s0 < ranges8[leaf].y; s0 += 8)
+        bs.push_back(int2{s0, std::min(8, ranges8[leaf].y - s0)});
+    }
+    st[d.leaves] = int(bs.size());
+    count = int(bs.size());
+    start = upload(P, st.data(), st.size());
+    src = upload(P, bs.data(), std::max<size_t>(bs.size(), 1));
+  };
+  blocks8_of(ls, d.blk8_start, d.blk8_src, d.blk8_count);
+  blocks8_of(lt, P->adj_blk8_start, P->adj_blk8_src, P->adj_blk8_count);
   std::vector<int2> adj_tiles;
   for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
     for (int r0 = 0; r0 < 3 * (ls[leaf].y - ls[leaf].x); r0 += 32) adj_tiles.push_back(int2{int(leaf), r0});
@@ -700,6 +720,12 @@ void ensure_capacity(hipfmm_plan* P, int K, bool potential, bool gradient = fals
   d.fields_out = alloc(points * 3 * cap, sizeof(S));
   d.sm = alloc(points * 3 * cap, sizeof(S));
   d.Hn = alloc(points * 3 * cap, sizeof(S));
+  d.smb = nullptr;
+  d.smb_blocks = 0;
+  if (sizeof(S) == 4 && cap >= 32) {  // bf16 moments for K a multiple of 32 up to cap
+    d.smb_blocks = std::max({d.blk8_count, P->adj_blk8_count, 1});
+    d.smb = alloc(size_t(d.smb_blocks) * 9 * 8 * (cap / 32 * 32), 2);
+  }
   d.Pn = nullptr;
   d.potentials_out = nullptr;
   if (P->potential_used) {
@@ -790,6 +816,9 @@ DevicePlan role_exchanged(const hipfmm_plan* P) {
   a.p2p_tile_count = P->adj_tile_count;
   a.p2p_tiles16 = P->adj_tiles16;
   a.p2p_tile16_count = P->adj_tile16_count;
+  a.blk8_start = P->adj_blk8_start;
+  a.blk8_src = P->adj_blk8_src;
+  a.blk8_count = P->adj_blk8_count;
   return a;
 }
 
@@ -1245,7 +1274,7 @@ int hipfmm_plan_set_launch_choices(hipfmm_plan* plan, int m2l_tile_shape, int p2
     const bool shape_ok = m2l_tile_shape == 0 || std::find(std::begin(hipfmm::kM2LShapes), std::end(hipfmm::kM2LShapes),
                                                            m2l_tile_shape) != std::end(hipfmm::kM2LShapes);
     require(shape_ok, "unknown M2L tile shape code");
-    require(p2p_kernel_variant >= 0 && p2p_kernel_variant <= 3, "unknown near-field kernel variant");
+    require(p2p_kernel_variant >= 0 && p2p_kernel_variant <= 4, "unknown near-field kernel variant");
     plan->d.m2l_shape = m2l_tile_shape;
     plan->d.p2p_variant = p2p_kernel_variant;
     plan->stats.m2l_tile_shape = uint64_t(m2l_tile_shape);

@@ -235,9 +235,11 @@ def test_group_rejects_duplicates_and_bad_batches(hip):
 
 @pytest.mark.parametrize("output", ["field", "both"])
 def test_near_field_variants_agree(hip, output):
-    # The three near-field kernels at K = 32 (vector ALU; matrix cores with
-    # (target, component) rows; matrix cores with target rows) against the
-    # FP64 FMM and each other, fields and potentials.
+    # The near-field kernels at K = 32 (vector ALU; matrix cores with
+    # (target, component) rows; matrix cores with target rows; matrix cores
+    # on bf16 three-piece operands, which hands potentials to the FP32
+    # target-rows kernel) against the FP64 FMM and each other, fields and
+    # potentials.
     rng = np.random.default_rng(980)
     n, K = 9000, 32
     x = rng.uniform(0, 1, size=(n, 3))
@@ -247,7 +249,7 @@ def test_near_field_variants_agree(hip, output):
     ref = O.CpuFmm(x, x, 4, 3, 64, fixed_ids=np.arange(n))
     want = [ref.evaluate(ms[k], np.arange(n), output="both") for k in (0, K - 1)]
     got = {}
-    for variant in (1, 2, 3):
+    for variant in (1, 2, 3, 4):
         f.set_launch_choices(0, variant)
         assert f.hip_plan_statistics["p2p_kernel_variant"] == variant
         if output == "field":
@@ -260,8 +262,41 @@ def test_near_field_variants_agree(hip, output):
             assert e <= 1e-5
             if output == "both":
                 assert rel_l2(got[variant]["phi"][k], want[i]["phi"]) <= 1e-5
-    for variant in (2, 3):
+    for variant in (2, 3, 4):
         assert rel_l2(got[variant]["H"], got[1]["H"]) <= 2e-6
+
+
+def test_bf16_three_piece_near_field_against_the_direct_sum(hip):
+    # Depth 1: every pair is a near pair, so the field is the P2P alone.  The
+    # three-piece bf16 products keep FP32 accuracy: against the FP64 direct
+    # sum like the FP32 matrix-core kernel, and the two kernels within 3e-7.
+    # Leaf sizes not a multiple of 8 or 32 exercise the padded source blocks,
+    # a repeated point outside the identity map the NaN of coincident pairs.
+    rng = np.random.default_rng(985)
+    n, K = 3001, 32
+    x = rng.uniform(0, 1, size=(n, 3))
+    x[7] = x[1000]
+    ids = np.arange(n)
+    ms = rng.normal(size=(K, n, 3))
+    announce("bf16 three-piece near field")
+    f = make(hip, x, x, 3, 1, 32, ids=ids)
+    got = {}
+    for variant in (3, 4):
+        f.set_launch_choices(0, variant)
+        got[variant] = f.evaluate_batch(ms)
+    bad = np.zeros(n, bool)
+    bad[[7, 1000]] = True
+    for variant in (3, 4):
+        assert np.isnan(got[variant][:, bad]).all() and np.isfinite(got[variant][:, ~bad]).all()
+    for k in (0, K - 1):
+        want = O.direct(x, x, ms[k], ids)["H"]
+        for variant in (3, 4):
+            e = rel_l2(got[variant][k][~bad], want[~bad])
+            print(f"  state {k} variant {variant}: vs FP64 direct {e:.3e}", flush=True)
+            assert e <= 1e-6
+    e = rel_l2(got[4][:, ~bad], got[3][:, ~bad])
+    print(f"  bf16 three-piece vs FP32 kernel {e:.3e}", flush=True)
+    assert e <= 3e-7
     with pytest.raises(ValueError, match="near-field kernel variant"):
         f.set_launch_choices(0, 7)
     with pytest.raises(ValueError, match="tile shape"):
