@@ -549,16 +549,18 @@ __device__ __forceinline__ void split3_bf16(const float (&x)[8], uint4& hi, uint
 }
 
 // The sorted moments as B operands of p2p_bf16_kernel: per block of 8
-// sources of one leaf, smb[block][c'][piece][k][8] (pieces hi, mid, lo of
-// m[s][c'][k]), zero past the leaf's last source.  Thread = (block, state);
-// reads are coalesced over the states, writes are 16-byte rows.
+// sources of one leaf and tile of 32 states,
+// smb[block][state tile][c'][piece][32][8] (pieces hi, mid, lo of
+// m[s][c'][k]), zero past the leaf's last source; a wave's 18 operand loads
+// of one block then sit at fixed offsets from one address.  Thread = (block,
+// state); reads are coalesced over the states, writes are 16-byte rows.
 __global__ void __launch_bounds__(256) split_moments_kernel(const DevicePlan d, int K) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= int64_t(d.blk8_count) * K) return;
   const int b = int(i / K), k = int(i % K);
   const int2 bs = d.blk8_src[b];
   const float* sm = static_cast<const float*>(d.sm);
-  uint4* out = static_cast<uint4*>(d.smb);
+  uint4* out = static_cast<uint4*>(d.smb) + ((int64_t(b) * (K / 32) + k / 32) * 9) * 32 + k % 32;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float x[8];
@@ -566,10 +568,9 @@ __global__ void __launch_bounds__(256) split_moments_kernel(const DevicePlan d, 
     for (int e = 0; e < 8; ++e) x[e] = e < bs.y ? sm[(int64_t(bs.x + e) * 3 + c) * K + k] : 0.0f;
     uint4 h, m, l;
     split3_bf16(x, h, m, l);
-    const int64_t at = (int64_t(b) * 3 + c) * 3 * K + k;
-    out[at] = h;
-    out[at + K] = m;
-    out[at + 2 * K] = l;
+    out[(3 * c) * 32] = h;
+    out[(3 * c + 1) * 32] = m;
+    out[(3 * c + 2) * 32] = l;
   }
 }
 
@@ -596,7 +597,8 @@ __device__ __forceinline__ void p2p_bf16_products(f32x4 (&part)[2], const uint4&
 
 // List-1 P2P on the matrix cores with bf16 operands
 // (v_mfma_f32_16x16x32_bf16), for FP32 plans with K a multiple of 32, fields
-// only.  Both operands are split in three bf16 pieces (hi + mid + lo, 24
+// only (the default there).  Capped at 168 registers, three waves per SIMD:
+// 5.8 ms against 6.1 ms uncapped at 1M points, p = 8, depth 5, K = 32.  Both operands are split in three bf16 pieces (hi + mid + lo, 24
 // significant bits) and the six piece products whose orders sum to at most 2
 // are kept, D m ~ Dh mh + Dh mm + Dm mh + Dh ml + Dl mh + Dm mm: the dropped
 // ones are below 2^-26 |D m|, under the rounding of FP32 operands.  Against
@@ -616,7 +618,8 @@ __device__ __forceinline__ void p2p_bf16_products(f32x4 (&part)[2], const uint4&
 // block, and its pairs, padding pairs and the identity pair have |r|^-1 = 0
 // and so zero entries.  Coincident pairs give NaN.  Blocked summation per 32
 // sources as in the FP32 kernels.
-__global__ void __launch_bounds__(64 * kP2PWaves) p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
+__global__ void __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(3)))
+p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
                                                                    int tile_count, int K) {
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
   const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * kP2PWaves + wave;
@@ -653,13 +656,14 @@ __global__ void __launch_bounds__(64 * kP2PWaves) p2p_bf16_kernel(const DevicePl
         for (int jb = 0; jb < n; jb += 32) {
           const int o0 = jb + 8 * g;
           const int blk = pb + (o0 < n ? (o0 >> 3) : 0);
+          const uint4* bp = mb + ((int64_t(blk) * (K / 32) + (kc0 >> 5)) * 9) * 32 + col;
           uint4 b[3][3][2];  // [component][piece][state half]
 #pragma unroll
           for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int pc = 0; pc < 3; ++pc)
 #pragma unroll
-              for (int h = 0; h < 2; ++h) b[c][pc][h] = mb[((int64_t(blk) * 3 + c) * 3 + pc) * K + kc0 + 16 * h + col];
+              for (int h = 0; h < 2; ++h) b[c][pc][h] = bp[(3 * c + pc) * 32 + 16 * h];
           float r[3][8], f[8], gf[8];
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
@@ -735,7 +739,7 @@ void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
   if (potential && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
   if constexpr (sizeof(S) == 4) {
-    if (K % 32 == 0 && d.p2p_variant == 4 && !potential) {
+    if (K % 32 == 0 && (d.p2p_variant == 4 || d.p2p_variant == 0) && !potential) {
       launch_p2p_bf16(d, K, s);
       return;
     }

@@ -58,7 +58,7 @@ def test_autotune_choices_meet_the_bar(hip, mode):
     st = f.hip_plan_statistics
     print(f"  shape {st['m2l_tile_shape']} near-field variant {st['p2p_kernel_variant']} "
           f"in {st['autotune_seconds']:.2f}s", flush=True)
-    assert st["m2l_tile_shape"] in SHAPES and st["p2p_kernel_variant"] in (1, 2, 3)
+    assert st["m2l_tile_shape"] in SHAPES and st["p2p_kernel_variant"] in (1, 2, 3, 4)
     after = f.evaluate_batch(ms)
     ref = O.CpuFmm(x, x, 6, 3, 64, fixed_ids=np.arange(n))
     for k in (0, K - 1):
