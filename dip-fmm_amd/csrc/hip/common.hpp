@@ -141,6 +141,8 @@ struct DevicePlan {
   int* blk8_start = nullptr;
   int2* blk8_src = nullptr;
   int blk8_count = 0;
+  int* nbr_blk_start = nullptr;  // per target leaf: its neighbourhood's blocks are
+  int* nbr_blk = nullptr;        // nbr_blk[nbr_blk_start[l] .. nbr_blk_start[l+1])
   void* smb = nullptr;         // bf16 [block][3 components][3 pieces][K][8 sources]
   int smb_blocks = 0;          // blocks smb holds (both roles)
   // basis tables

@@ -606,18 +606,19 @@ __device__ __forceinline__ void p2p_bf16_products(f32x4 (&part)[2], const uint4&
 // bf16 products issue 2.7x fewer matrix-core cycles per pair.
 //
 // One wave owns 16 targets of one leaf and 32 states; the depth of a product
-// is 32 sources of one neighbour leaf, and lane (row i, group g) forms the
-// pairs of target i with that block's sources 8g..8g+7: their offsets r, then
+// is four blocks of 8 sources from the list of the neighbourhood's blocks,
+// and lane (row i, group g) forms the pairs of target i with the 8 sources
+// of block g: their offsets r, then
 // the 6 distinct entries D_ab = 3 f r_a r_b / |r|^2 - delta_ab f with
 // f = |r|^-3 / (4 pi), each split into pieces for the products of both
 // components it couples.  Lane layout (CDNA4 16x16x32 bf16): A[i][k] in lane
 // i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8;
 // D[i][j] in lane j + 16 (i / 4), register i % 4.  The moments' pieces come
 // from split_moments_kernel in 8-source blocks, so a lane's B operand is one
-// 16-byte load; a group past the leaf's last source reads the leaf's first
+// 16-byte load; a group past the list's end reads the product's first
 // block, and its pairs, padding pairs and the identity pair have |r|^-1 = 0
-// and so zero entries.  Coincident pairs give NaN.  Blocked summation per 32
-// sources as in the FP32 kernels.
+// and so zero entries.  Coincident pairs give NaN.  Blocked summation per
+// product (32 source slots) as in the FP32 kernels.
 __global__ void __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(3)))
 p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
                                                                    int tile_count, int K) {
@@ -639,73 +640,63 @@ p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int h = 0; h < 2; ++h) acc[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  const int nb = 1 << d.L;
-  int lx, ly, lz;
-  demorton3(uint32_t(leaf), d.L, lx, ly, lz);
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = lx + dx, y = ly + dy, z = lz + dz;
-        if (x < 0 || y < 0 || z < 0 || x >= nb || y >= nb || z >= nb) continue;
-        const uint32_t nl = morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L);
-        const int2 sr = d.leaf_src[nl];
-        const int n = sr.y - sr.x;
-        const int pb = d.blk8_start[nl];
-        const float4* shi = static_cast<const float4*>(d.src_pos) + sr.x;
-        const float4* slo = shi + d.ns;
-        for (int jb = 0; jb < n; jb += 32) {
-          const int o0 = jb + 8 * g;
-          const int blk = pb + (o0 < n ? (o0 >> 3) : 0);
-          const uint4* bp = mb + ((int64_t(blk) * (K / 32) + (kc0 >> 5)) * 9) * 32 + col;
-          uint4 b[3][3][2];  // [component][piece][state half]
+  // The neighbourhood's 8-source blocks in one list, four per product (one
+  // per lane group): a leaf pads to 8 sources rather than to 32.
+  const int q0 = d.nbr_blk_start[leaf], q1 = d.nbr_blk_start[leaf + 1];
+  const float4* shi = static_cast<const float4*>(d.src_pos);
+  const float4* slo = shi + d.ns;
+  for (int q = q0; q < q1; q += 4) {
+    const bool grp_ok = q + g < q1;
+    const int blk = d.nbr_blk[grp_ok ? q + g : q];
+    const int2 bs = d.blk8_src[blk];
+    const int cnt = grp_ok ? bs.y : 0;
+    const uint4* bp = mb + ((int64_t(blk) * (K / 32) + (kc0 >> 5)) * 9) * 32 + col;
+    uint4 b[3][3][2];  // [component][piece][state half]
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int pc = 0; pc < 3; ++pc)
+      for (int pc = 0; pc < 3; ++pc)
 #pragma unroll
-              for (int h = 0; h < 2; ++h) b[c][pc][h] = bp[(3 * c + pc) * 32 + 16 * h];
-          float r[3][8], f[8], gf[8];
+        for (int h = 0; h < 2; ++h) b[c][pc][h] = bp[(3 * c + pc) * 32 + 16 * h];
+    float r[3][8], f[8], gf[8];
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int o = o0 + e;
-            const uint32_t j = o < n ? uint32_t(o) : 0u;
-            const PairPos<float> sp{shi[j], slo[j]};
-            tp.diff(sp, r[0][e], r[1][e], r[2][e]);
-            const float r2 = r[0][e] * r[0][e] + r[1][e] * r[1][e] + r[2][e] * r[2][e];
-            const bool use = row_ok & (o < n) & (sr.x + o != self);
-            const float rinv = use ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-            const float ri2 = rinv * rinv;
-            f[e] = ri2 * rinv * 0.079577471545947667884441881686257181f;
-            gf[e] = 3.0f * f[e] * ri2;
-          }
-          f32x4 part[3][2];
+    for (int e = 0; e < 8; ++e) {
+      const int sidx = bs.x + (e < cnt ? e : 0);
+      const PairPos<float> sp{shi[sidx], slo[sidx]};
+      tp.diff(sp, r[0][e], r[1][e], r[2][e]);
+      const float r2 = r[0][e] * r[0][e] + r[1][e] * r[1][e] + r[2][e] * r[2][e];
+      const bool use = row_ok & (e < cnt) & (bs.x + e != self);
+      const float rinv = use ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+      const float ri2 = rinv * rinv;
+      f[e] = ri2 * rinv * 0.079577471545947667884441881686257181f;
+      gf[e] = 3.0f * f[e] * ri2;
+    }
+    f32x4 part[3][2];
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-          for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int bc = a; bc < 3; ++bc) {
-              float v[8];
+      for (int bc = a; bc < 3; ++bc) {
+        float v[8];
 #pragma unroll
-              for (int e = 0; e < 8; ++e) v[e] = gf[e] * r[a][e] * r[bc][e] - (a == bc ? f[e] : 0.0f);
-              uint4 ph, pm, pl;
-              split3_bf16(v, ph, pm, pl);
-              const uint4 bb[3][2] = {{b[bc][0][0], b[bc][0][1]}, {b[bc][1][0], b[bc][1][1]},
-                                      {b[bc][2][0], b[bc][2][1]}};
-              p2p_bf16_products(part[a], ph, pm, pl, bb);
-              if (a != bc) {
-                const uint4 ba[3][2] = {{b[a][0][0], b[a][0][1]}, {b[a][1][0], b[a][1][1]}, {b[a][2][0], b[a][2][1]}};
-                p2p_bf16_products(part[bc], ph, pm, pl, ba);
-              }
-            }
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) acc[c][h] += part[c][h];
+        for (int e = 0; e < 8; ++e) v[e] = gf[e] * r[a][e] * r[bc][e] - (a == bc ? f[e] : 0.0f);
+        uint4 ph, pm, pl;
+        split3_bf16(v, ph, pm, pl);
+        const uint4 bb[3][2] = {{b[bc][0][0], b[bc][0][1]}, {b[bc][1][0], b[bc][1][1]}, {b[bc][2][0], b[bc][2][1]}};
+        p2p_bf16_products(part[a], ph, pm, pl, bb);
+        if (a != bc) {
+          const uint4 ba[3][2] = {{b[a][0][0], b[a][0][1]}, {b[a][1][0], b[a][1][1]}, {b[a][2][0], b[a][2][1]}};
+          p2p_bf16_products(part[bc], ph, pm, pl, ba);
         }
       }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) acc[c][h] += part[c][h];
+  }
   // acc[c][h][i]: target tile.y + 4g + i, state kc0 + 16h + col.
   float* Hn = static_cast<float*>(d.Hn);
 #pragma unroll
@@ -725,10 +716,12 @@ void launch_p2p_bf16(const DevicePlan& d, int K, hipStream_t s) {
   if (d.smb == nullptr || d.smb_blocks < d.blk8_count)
     throw invalid_argument("bf16 near-field moment buffer is not allocated");
   const int count = d.p2p_tile16_count;
-  if (count == 0 || d.blk8_count == 0) return;
-  const int64_t threads = int64_t(d.blk8_count) * K;
-  split_moments_kernel<<<unsigned((threads + 255) / 256), 256, 0, s>>>(d, K);
-  HIPFMM_CHECK(hipGetLastError());
+  if (count == 0) return;
+  if (d.blk8_count > 0) {  // without sources the kernel writes zero fields
+    const int64_t threads = int64_t(d.blk8_count) * K;
+    split_moments_kernel<<<unsigned((threads + 255) / 256), 256, 0, s>>>(d, K);
+    HIPFMM_CHECK(hipGetLastError());
+  }
   const dim3 grid(unsigned((count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
   p2p_bf16_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, count, K);
   HIPFMM_CHECK(hipGetLastError());

@@ -70,6 +70,8 @@ struct hipfmm_plan {
   int* adj_blk8_start = nullptr;
   int2* adj_blk8_src = nullptr;
   int adj_blk8_count = 0;
+  int* adj_nbr_blk_start = nullptr;
+  int* adj_nbr_blk = nullptr;
   int* adj_self = nullptr;
   bool adj_self_valid = true;  // false when two targets share one source
   bool potential_used = false;  // keep the near-field potential buffers in the state
@@ -357,8 +359,11 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
   d.p2p_tiles16 = tiles16_of(lt, d.p2p_tile16_count);
   P->adj_tiles16 = tiles16_of(ls, P->adj_tile16_count);
   // Source blocks of 8 of the bf16 three-piece near field, per source leaf
-  // (the adjoint's source leaves are the target leaves).
-  auto blocks8_of = [&](const std::vector<int2>& ranges8, int*& start, int2*& src, int& count) {
+  // (the adjoint's source leaves are the target leaves), and per target leaf
+  // the blocks of its 3x3x3 neighbourhood in one list (neighbours in the
+  // order dz, dy, dx from -1 to 1, each leaf's blocks in source order).
+  auto blocks8_of = [&](const std::vector<int2>& ranges8, int*& start, int2*& src, int& count, int*& nbr_start,
+                        int*& nbr) {
     std::vector<int> st(d.leaves + 1);
     std::vector<int2> bs;
     for (int64_t leaf = 0; leaf < d.leaves; ++leaf) {
@@ -370,9 +375,27 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
     count = int(bs.size());
     start = upload(P, st.data(), st.size());
     src = upload(P, bs.data(), std::max<size_t>(bs.size(), 1));
+    std::vector<int> ns(d.leaves + 1), nb;
+    const int side = 1 << d.L;
+    for (int64_t leaf = 0; leaf < d.leaves; ++leaf) {
+      ns[leaf] = int(nb.size());
+      int lx, ly, lz;
+      hipfmm::demorton3(uint32_t(leaf), d.L, lx, ly, lz);
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int x = lx + dx, y = ly + dy, z = lz + dz;
+            if (x < 0 || y < 0 || z < 0 || x >= side || y >= side || z >= side) continue;
+            const uint32_t nl = hipfmm::morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L);
+            for (int b = st[nl]; b < st[nl + 1]; ++b) nb.push_back(b);
+          }
+    }
+    ns[d.leaves] = int(nb.size());
+    nbr_start = upload(P, ns.data(), ns.size());
+    nbr = upload(P, nb.data(), std::max<size_t>(nb.size(), 1));
   };
-  blocks8_of(ls, d.blk8_start, d.blk8_src, d.blk8_count);
-  blocks8_of(lt, P->adj_blk8_start, P->adj_blk8_src, P->adj_blk8_count);
+  blocks8_of(ls, d.blk8_start, d.blk8_src, d.blk8_count, d.nbr_blk_start, d.nbr_blk);
+  blocks8_of(lt, P->adj_blk8_start, P->adj_blk8_src, P->adj_blk8_count, P->adj_nbr_blk_start, P->adj_nbr_blk);
   std::vector<int2> adj_tiles;
   for (int64_t leaf = 0; leaf < d.leaves; ++leaf)
     for (int r0 = 0; r0 < 3 * (ls[leaf].y - ls[leaf].x); r0 += 32) adj_tiles.push_back(int2{int(leaf), r0});
@@ -819,6 +842,8 @@ DevicePlan role_exchanged(const hipfmm_plan* P) {
   a.blk8_start = P->adj_blk8_start;
   a.blk8_src = P->adj_blk8_src;
   a.blk8_count = P->adj_blk8_count;
+  a.nbr_blk_start = P->adj_nbr_blk_start;
+  a.nbr_blk = P->adj_nbr_blk;
   return a;
 }
 

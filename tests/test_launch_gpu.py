@@ -269,7 +269,8 @@ def test_near_field_variants_agree(hip, output):
 def test_bf16_three_piece_near_field_against_the_direct_sum(hip):
     # Depth 1: every pair is a near pair, so the field is the P2P alone.  The
     # three-piece bf16 products keep FP32 accuracy: against the FP64 direct
-    # sum like the FP32 matrix-core kernel, and the two kernels within 3e-7.
+    # sum like the FP32 matrix-core kernel (both near 3e-7), and the two
+    # kernels, whose FP32 roundings are independent, within the same 1e-6.
     # Leaf sizes not a multiple of 8 or 32 exercise the padded source blocks,
     # a repeated point outside the identity map the NaN of coincident pairs.
     rng = np.random.default_rng(985)
@@ -296,7 +297,7 @@ def test_bf16_three_piece_near_field_against_the_direct_sum(hip):
             assert e <= 1e-6
     e = rel_l2(got[4][:, ~bad], got[3][:, ~bad])
     print(f"  bf16 three-piece vs FP32 kernel {e:.3e}", flush=True)
-    assert e <= 3e-7
+    assert e <= 1e-6
     with pytest.raises(ValueError, match="near-field kernel variant"):
         f.set_launch_choices(0, 7)
     with pytest.raises(ValueError, match="tile shape"):
