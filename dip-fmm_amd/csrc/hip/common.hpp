@@ -212,9 +212,8 @@ struct DevicePlan {
   // Launch choices (set by the autotuner; 0 = built-in default).
   int m2l_shape = 0;    // MFMA M2L tile shape code (kM2LShapes)
   int p2p_variant = 0;  // near field at K % 32 == 0 (FP32): 1 vector ALU, 2 matrix cores with
-                        // (target, component) rows, 3 matrix cores with target rows, 4 matrix
-                        // cores on bf16 three-piece operands (fields only); 0 = 4 for fields,
-                        // 3 with potentials
+                        // (target, component) rows, 3 matrix cores with target rows, 4 (= default)
+                        // matrix cores on bf16 three-piece operands
 };
 
 // MFMA M2L tile shapes of the harmonic form (<= 3 row tiles): the code is

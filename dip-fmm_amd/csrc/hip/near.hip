@@ -596,9 +596,11 @@ __device__ __forceinline__ void p2p_bf16_products(f32x4 (&part)[2], const uint4&
 }
 
 // List-1 P2P on the matrix cores with bf16 operands
-// (v_mfma_f32_16x16x32_bf16), for FP32 plans with K a multiple of 32, fields
-// only (the default there).  Capped at 168 registers, three waves per SIMD:
-// 5.8 ms against 6.1 ms uncapped at 1M points, p = 8, depth 5, K = 32.  Both operands are split in three bf16 pieces (hi + mid + lo, 24
+// (v_mfma_f32_16x16x32_bf16), for FP32 plans with K a multiple of 32 (the
+// default there).  Fields are capped at 168 registers, three waves per SIMD
+// (5.8 ms against 6.1 ms uncapped at 1M points, p = 8, depth 5, K = 32); with
+// the potential (kPot: a fourth accumulator row, phi += f r . m from the
+// entries f r_c') two waves, where the cap would spill.  Both operands are split in three bf16 pieces (hi + mid + lo, 24
 // significant bits) and the six piece products whose orders sum to at most 2
 // are kept, D m ~ Dh mh + Dh mm + Dm mh + Dh ml + Dl mh + Dm mm: the dropped
 // ones are below 2^-26 |D m|, under the rounding of FP32 operands.  Against
@@ -619,7 +621,8 @@ __device__ __forceinline__ void p2p_bf16_products(f32x4 (&part)[2], const uint4&
 // block, and its pairs, padding pairs and the identity pair have |r|^-1 = 0
 // and so zero entries.  Coincident pairs give NaN.  Blocked summation per
 // product (32 source slots) as in the FP32 kernels.
-__global__ void __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(3)))
+template <bool kPot>
+__global__ void __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(kPot ? 2 : 3)))
 p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
                                                                    int tile_count, int K) {
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
@@ -635,9 +638,10 @@ p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
   const int self = row_ok ? d.self[t] : -2;
   const int kc0 = int(blockIdx.y) * 32;
   const uint4* mb = static_cast<const uint4*>(d.smb);
-  f32x4 acc[3][2];
+  constexpr int kRows = kPot ? 4 : 3;  // field components, then the potential
+  f32x4 acc[kRows][2];
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
+  for (int c = 0; c < kRows; ++c)
 #pragma unroll
     for (int h = 0; h < 2; ++h) acc[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   // The neighbourhood's 8-source blocks in one list, four per product (one
@@ -671,9 +675,9 @@ p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
       f[e] = ri2 * rinv * 0.079577471545947667884441881686257181f;
       gf[e] = 3.0f * f[e] * ri2;
     }
-    f32x4 part[3][2];
+    f32x4 part[kRows][2];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < kRows; ++c)
 #pragma unroll
       for (int h = 0; h < 2; ++h) part[c][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -692,8 +696,21 @@ p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
           p2p_bf16_products(part[bc], ph, pm, pl, ba);
         }
       }
+    if constexpr (kPot) {
+      // phi += f (r . m): the entries f r_c' against the moments' component c'.
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+      for (int cp = 0; cp < 3; ++cp) {
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = f[e] * r[cp][e];
+        uint4 ph, pm, pl;
+        split3_bf16(v, ph, pm, pl);
+        const uint4 bb[3][2] = {{b[cp][0][0], b[cp][0][1]}, {b[cp][1][0], b[cp][1][1]}, {b[cp][2][0], b[cp][2][1]}};
+        p2p_bf16_products(part[3], ph, pm, pl, bb);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kRows; ++c)
 #pragma unroll
       for (int h = 0; h < 2; ++h) acc[c][h] += part[c][h];
   }
@@ -708,11 +725,12 @@ p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
       const int k = kc0 + 16 * h + col;
 #pragma unroll
       for (int c = 0; c < 3; ++c) Hn[(ot * 3 + c) * K + k] = acc[c][h][i];
+      if constexpr (kPot) static_cast<float*>(d.Pn)[ot * K + k] = acc[3][h][i];
     }
   }
 }
 
-void launch_p2p_bf16(const DevicePlan& d, int K, hipStream_t s) {
+void launch_p2p_bf16(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   if (d.smb == nullptr || d.smb_blocks < d.blk8_count)
     throw invalid_argument("bf16 near-field moment buffer is not allocated");
   const int count = d.p2p_tile16_count;
@@ -723,7 +741,10 @@ void launch_p2p_bf16(const DevicePlan& d, int K, hipStream_t s) {
     HIPFMM_CHECK(hipGetLastError());
   }
   const dim3 grid(unsigned((count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
-  p2p_bf16_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, count, K);
+  if (potential)
+    p2p_bf16_kernel<true><<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, count, K);
+  else
+    p2p_bf16_kernel<false><<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, count, K);
   HIPFMM_CHECK(hipGetLastError());
 }
 
@@ -732,8 +753,8 @@ void launch_p2p(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   if (d.leaves == 0 || d.nt == 0) return;
   if (potential && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
   if constexpr (sizeof(S) == 4) {
-    if (K % 32 == 0 && (d.p2p_variant == 4 || d.p2p_variant == 0) && !potential) {
-      launch_p2p_bf16(d, K, s);
+    if (K % 32 == 0 && (d.p2p_variant == 4 || d.p2p_variant == 0)) {
+      launch_p2p_bf16(d, K, potential, s);
       return;
     }
     if (K % 32 == 0 && d.p2p_variant != 1 && d.p2p_variant != 2) {

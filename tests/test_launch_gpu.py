@@ -237,9 +237,8 @@ def test_group_rejects_duplicates_and_bad_batches(hip):
 def test_near_field_variants_agree(hip, output):
     # The near-field kernels at K = 32 (vector ALU; matrix cores with
     # (target, component) rows; matrix cores with target rows; matrix cores
-    # on bf16 three-piece operands, which hands potentials to the FP32
-    # target-rows kernel) against the FP64 FMM and each other, fields and
-    # potentials.
+    # on bf16 three-piece operands) against the FP64 FMM and each other,
+    # fields and potentials.
     rng = np.random.default_rng(980)
     n, K = 9000, 32
     x = rng.uniform(0, 1, size=(n, 3))
