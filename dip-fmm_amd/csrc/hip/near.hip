@@ -730,16 +730,20 @@ p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
   }
 }
 
-void launch_p2p_bf16(const DevicePlan& d, int K, bool potential, hipStream_t s) {
+// The moments' bf16 pieces for the near-field kernels on bf16 operands.
+void launch_split_moments(const DevicePlan& d, int K, hipStream_t s) {
   if (d.smb == nullptr || d.smb_blocks < d.blk8_count)
     throw invalid_argument("bf16 near-field moment buffer is not allocated");
+  if (d.blk8_count == 0) return;  // without sources the near-field kernels write zeros
+  const int64_t threads = int64_t(d.blk8_count) * K;
+  split_moments_kernel<<<unsigned((threads + 255) / 256), 256, 0, s>>>(d, K);
+  HIPFMM_CHECK(hipGetLastError());
+}
+
+void launch_p2p_bf16(const DevicePlan& d, int K, bool potential, hipStream_t s) {
   const int count = d.p2p_tile16_count;
   if (count == 0) return;
-  if (d.blk8_count > 0) {  // without sources the kernel writes zero fields
-    const int64_t threads = int64_t(d.blk8_count) * K;
-    split_moments_kernel<<<unsigned((threads + 255) / 256), 256, 0, s>>>(d, K);
-    HIPFMM_CHECK(hipGetLastError());
-  }
+  launch_split_moments(d, K, s);
   const dim3 grid(unsigned((count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
   if (potential)
     p2p_bf16_kernel<true><<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, count, K);
@@ -902,6 +906,141 @@ void launch_p2p_grad_t(const DevicePlan& d, int K, hipStream_t s) {
 
 }  // namespace
 
+// acc[E] += W m_D over both state halves, for one entry of W (its pieces)
+// and the moments' component D.
+template <int E, int D>
+__device__ __forceinline__ void grad_bf16_products(f32x4 (&acc)[6][2], const uint4& ph, const uint4& pm,
+                                                   const uint4& pl, const uint4 (&b)[3][3][2]) {
+  const uint4 bb[3][2] = {{b[D][0][0], b[D][0][1]}, {b[D][1][0], b[D][1][1]}, {b[D][2][0], b[D][2][1]}};
+  p2p_bf16_products(acc[E], ph, pm, pl, bb);
+}
+
+// Near-field gradient on bf16 three-piece operands (FP32 plans, K a multiple
+// of 32; the default there): the tiles, the neighbourhood's block list and
+// the operand layout of p2p_bf16_kernel, with the 10 distinct entries of the
+// symmetric 3-tensor W_cjd = g (delta_cj u_d + delta_jd u_c + delta_cd u_j -
+// 5 u_c u_j u_d), g = 3/(4 pi |r|^4), each split once and multiplied by the
+// moments' components it meets in G_cj += sum_d W_cjd m_d: 18 (output,
+// component) pairs of six bf16 products per state half.  Two waves per SIMD;
+// accumulation straight into the totals, as in the FP32 gradient kernel.
+// Padding pairs and the identity pair have |r|^-1 = 0; coincident pairs give
+// NaN.  Writes Gn[t][6][K] (xx xy xz yy yz zz).
+__global__ void __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(2)))
+p2p_grad_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles, int tile_count, int K) {
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * kP2PWaves + wave;
+  if (item >= tile_count) return;
+  const int2 tile = tiles[item];
+  const int leaf = tile.x;
+  const int2 trange = d.leaf_tgt[leaf];
+  const int col = lane & 15, g = lane >> 4;
+  const bool row_ok = trange.x + tile.y + col < trange.y;
+  const int t = row_ok ? trange.x + tile.y + col : trange.x;
+  const PairPos<float> tp = PairPos<float>::load(d.tgt_pos, d.nt, t);
+  const int self = row_ok ? d.self[t] : -2;
+  const int kc0 = int(blockIdx.y) * 32;
+  const uint4* mb = static_cast<const uint4*>(d.smb);
+  f32x4 acc[6][2];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) acc[e][0] = acc[e][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const float c34pi = 0.23873241463784300365332564505877154f;  // 3/(4 pi)
+  const int q0 = d.nbr_blk_start[leaf], q1 = d.nbr_blk_start[leaf + 1];
+  const float4* shi = static_cast<const float4*>(d.src_pos);
+  const float4* slo = shi + d.ns;
+  for (int q = q0; q < q1; q += 4) {
+    const bool grp_ok = q + g < q1;
+    const int blk = d.nbr_blk[grp_ok ? q + g : q];
+    const int2 bs = d.blk8_src[blk];
+    const int cnt = grp_ok ? bs.y : 0;
+    const uint4* bp = mb + ((int64_t(blk) * (K / 32) + (kc0 >> 5)) * 9) * 32 + col;
+    uint4 b[3][3][2];  // [component][piece][state half]
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int pc = 0; pc < 3; ++pc)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) b[c][pc][h] = bp[(3 * c + pc) * 32 + 16 * h];
+    float ux[8], uy[8], uz[8], gg[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int sidx = bs.x + (e < cnt ? e : 0);
+      const PairPos<float> sp{shi[sidx], slo[sidx]};
+      float rx, ry, rz;
+      tp.diff(sp, rx, ry, rz);
+      const float r2 = rx * rx + ry * ry + rz * rz;
+      const bool use = row_ok & (e < cnt) & (bs.x + e != self);
+      const float rinv = use ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+      ux[e] = rx * rinv;
+      uy[e] = ry * rinv;
+      uz[e] = rz * rinv;
+      gg[e] = c34pi * rinv * rinv * rinv * rinv;
+    }
+    uint4 ph, pm, pl;
+    float v[8];
+    // W_cjd with sorted indices 000 001 002 011 012 022 111 112 122 222, and
+    // the (output cj, component d) pairs each one serves.
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = gg[e] * ux[e] * (3.0f - 5.0f * ux[e] * ux[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<0, 0>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = uy[e] * gg[e] * (1.0f - 5.0f * ux[e] * ux[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<0, 1>(acc, ph, pm, pl, b);
+    grad_bf16_products<1, 0>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = uz[e] * gg[e] * (1.0f - 5.0f * ux[e] * ux[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<0, 2>(acc, ph, pm, pl, b);
+    grad_bf16_products<2, 0>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = ux[e] * gg[e] * (1.0f - 5.0f * uy[e] * uy[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<1, 1>(acc, ph, pm, pl, b);
+    grad_bf16_products<3, 0>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = -5.0f * gg[e] * ux[e] * uy[e] * uz[e];
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<1, 2>(acc, ph, pm, pl, b);
+    grad_bf16_products<2, 1>(acc, ph, pm, pl, b);
+    grad_bf16_products<4, 0>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = ux[e] * gg[e] * (1.0f - 5.0f * uz[e] * uz[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<2, 2>(acc, ph, pm, pl, b);
+    grad_bf16_products<5, 0>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = gg[e] * uy[e] * (3.0f - 5.0f * uy[e] * uy[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<3, 1>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = uz[e] * gg[e] * (1.0f - 5.0f * uy[e] * uy[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<3, 2>(acc, ph, pm, pl, b);
+    grad_bf16_products<4, 1>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = uy[e] * gg[e] * (1.0f - 5.0f * uz[e] * uz[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<4, 2>(acc, ph, pm, pl, b);
+    grad_bf16_products<5, 1>(acc, ph, pm, pl, b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = gg[e] * uz[e] * (3.0f - 5.0f * uz[e] * uz[e]);
+    split3_bf16(v, ph, pm, pl);
+    grad_bf16_products<5, 2>(acc, ph, pm, pl, b);
+  }
+  // acc[e][h][i]: target tile.y + 4g + i, state kc0 + 16h + col.
+  float* Gn = static_cast<float*>(d.Gn);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t ot = int64_t(trange.x) + tile.y + 4 * g + i;
+    if (ot >= trange.y) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int e = 0; e < 6; ++e) Gn[(ot * 6 + e) * K + kc0 + 16 * h + col] = acc[e][h][i];
+  }
+}
+
 // Near-field gradient on the matrix cores (FP32 plans, K a multiple of 32),
 // targets as rows like p2p_mfma16_kernel: with g = 3/(4 pi |r|^4) the
 // derivative of the pair tensor is the fully symmetric 3-tensor
@@ -1000,7 +1139,12 @@ void launch_p2p_grad(const DevicePlan& d, int K, hipStream_t s) {
   if constexpr (sizeof(S) == 4) {
     if (K % 32 == 0 && d.p2p_tile16_count > 0) {
       const dim3 grid(unsigned((d.p2p_tile16_count + kP2PWaves - 1) / kP2PWaves), unsigned(K / 32));
-      p2p_grad_mfma16_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, d.p2p_tile16_count, K);
+      if (d.p2p_variant == 0 || d.p2p_variant == 4) {
+        launch_split_moments(d, K, s);
+        p2p_grad_bf16_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, d.p2p_tile16_count, K);
+      } else {
+        p2p_grad_mfma16_kernel<<<grid, 64 * kP2PWaves, 0, s>>>(d, d.p2p_tiles16, d.p2p_tile16_count, K);
+      }
       HIPFMM_CHECK(hipGetLastError());
       return;
     }
