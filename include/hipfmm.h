@@ -144,7 +144,9 @@ typedef struct hipfmm_stats {
    * reference): the matrix-core M2L tile shape code, 10 * column tiles per
    * wave + ring stages (0 = built-in default), and the near-field kernel at
    * batch % 32 == 0 (0 default, 1 vector ALU, 2 matrix cores with (target,
-   * component) rows, 3 matrix cores with target rows -- the default). */
+   * component) rows, 3 matrix cores with target rows, 4 matrix cores on bf16
+   * three-piece operands -- the default; the field gradient runs the bf16
+   * kernel for 0 and 4 and the FP32 target-rows kernel otherwise). */
   uint64_t m2l_tile_shape;
   uint64_t p2p_kernel_variant;
   double autotune_seconds; /* host time of the last hipfmm_plan_autotune */
@@ -262,16 +264,17 @@ int hipfmm_plan_set_m2l_mode(hipfmm_plan* plan, int mode);
 /* Launch autotuner; no counterpart in the reference (whose CudaFull launch
  * shapes are fixed, src/cuda_fmm.cu:3500-3600).  Times, on the plan's own
  * state buffers at `batch` states and the current M2L mode, every tile shape
- * of the matrix-core M2L and both near-field kernels (FP32 plans, batch a
- * multiple of 32), keeps the fastest of each for later evaluates of this
- * plan and reports them in hipfmm_stats.  Results are unchanged in value
+ * of the matrix-core M2L (by its kernel time) and the four near-field kernels
+ * (FP32 plans, batch a multiple of 32; by the whole evaluate, since the near
+ * field overlaps the far chain), keeps the fastest of each for later
+ * evaluates of this plan and reports them in hipfmm_stats.  Results are unchanged in value
  * class: every candidate meets the same accuracy bars.  Overwrites the
  * plan's coefficients of the last evaluate. */
 int hipfmm_plan_autotune(hipfmm_plan* plan, int batch);
 
 /* Set the launch choices directly (values as reported in hipfmm_stats: a
  * tile shape code 23, 13, 43, 53, 24, 34 or 26, or 0 for the default; a near-field
- * variant 0..3), e.g. to reuse an earlier autotune result.  No counterpart in
+ * variant 0..4), e.g. to reuse an earlier autotune result.  No counterpart in
  * the reference. */
 int hipfmm_plan_set_launch_choices(hipfmm_plan* plan, int m2l_tile_shape, int p2p_kernel_variant);
 

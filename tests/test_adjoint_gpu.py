@@ -122,6 +122,25 @@ def test_batched_adjoint_on_matrix_cores(hip, mode):
         assert e <= 1e-5
 
 
+def test_batched_adjoint_of_separate_points_on_matrix_cores(hip):
+    # Sources and targets apart at K = 32 on an FP32 plan: the adjoint's near
+    # field walks the block lists of the role-exchanged view (the blocks of
+    # the target leaves around each source leaf).
+    rng = np.random.default_rng(716)
+    src, tgt = rng.uniform(0, 1, size=(5000, 3)), rng.uniform(0.05, 0.95, size=(3500, 3))
+    announce("batched adjoint, separate points")
+    f = make(hip, src, tgt, 5, 3, 32, mode=hip.HipM2LMode.MFMA_BF16X3)
+    y = rng.normal(size=(32, len(tgt), 3))
+    aty = f.evaluate_adjoint_batch(y)
+    assert aty.shape == (32, len(src), 3)
+    root = root_of(f)
+    ref = O.CpuFmm(tgt, src, 5, 3, 64, root_centre=root[:3], root_half_width=root[3])
+    for k in (0, 31):
+        e = rel_l2(aty[k], ref.evaluate(y[k])["H"])
+        print(f"  state {k}: rel L2 vs exchanged FP64 FMM {e:.3e}", flush=True)
+        assert e <= 1e-5
+
+
 def test_adjoint_identity_map_is_transposed(hip):
     # Targets = sources with a shuffled identity map: the adjoint skips the
     # pair (source s, target t) exactly where the forward skips (t, s).

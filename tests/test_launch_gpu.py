@@ -265,7 +265,8 @@ def test_near_field_variants_agree(hip, output):
         assert rel_l2(got[variant]["H"], got[1]["H"]) <= 2e-6
 
 
-def test_bf16_three_piece_near_field_against_the_direct_sum(hip):
+@pytest.mark.parametrize("K", [32, 64])
+def test_bf16_three_piece_near_field_against_the_direct_sum(hip, K):
     # Depth 1: every pair is a near pair, so the field is the P2P alone.  The
     # three-piece bf16 products keep FP32 accuracy: against the FP64 direct
     # sum like the FP32 matrix-core kernel (both near 3e-7), and the two
@@ -273,12 +274,12 @@ def test_bf16_three_piece_near_field_against_the_direct_sum(hip):
     # Leaf sizes not a multiple of 8 or 32 exercise the padded source blocks,
     # a repeated point outside the identity map the NaN of coincident pairs.
     rng = np.random.default_rng(985)
-    n, K = 3001, 32
+    n = 3001
     x = rng.uniform(0, 1, size=(n, 3))
     x[7] = x[1000]
     ids = np.arange(n)
     ms = rng.normal(size=(K, n, 3))
-    announce("bf16 three-piece near field")
+    announce("bf16 three-piece near field", K)
     f = make(hip, x, x, 3, 1, 32, ids=ids)
     got = {}
     for variant in (3, 4):
