@@ -584,15 +584,19 @@ __device__ __forceinline__ f32x4 mfma_bf16(const uint4& a, const uint4& b, const
 // sum to at most 2, smallest first.
 __device__ __forceinline__ void p2p_bf16_products(f32x4 (&part)[2], const uint4& ah, const uint4& am, const uint4& al,
                                                   const uint4 (&b)[3][2]) {
+  // The two halves' chains alternate, so consecutive products are independent.
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    part[h] = mfma_bf16(am, b[1][h], part[h]);
-    part[h] = mfma_bf16(al, b[0][h], part[h]);
-    part[h] = mfma_bf16(ah, b[2][h], part[h]);
-    part[h] = mfma_bf16(am, b[0][h], part[h]);
-    part[h] = mfma_bf16(ah, b[1][h], part[h]);
-    part[h] = mfma_bf16(ah, b[0][h], part[h]);
-  }
+  for (int h = 0; h < 2; ++h) part[h] = mfma_bf16(am, b[1][h], part[h]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) part[h] = mfma_bf16(al, b[0][h], part[h]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) part[h] = mfma_bf16(ah, b[2][h], part[h]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) part[h] = mfma_bf16(am, b[0][h], part[h]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) part[h] = mfma_bf16(ah, b[1][h], part[h]);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) part[h] = mfma_bf16(ah, b[0][h], part[h]);
 }
 
 // List-1 P2P on the matrix cores with bf16 operands

@@ -389,6 +389,7 @@ void create_impl(hipfmm_plan* P, const hipfmm_plan_desc& desc) {
             const uint32_t nl = hipfmm::morton3(uint32_t(x), uint32_t(y), uint32_t(z), d.L);
             for (int b = st[nl]; b < st[nl + 1]; ++b) nb.push_back(b);
           }
+      require(nb.size() < size_t(INT32_MAX), "near-field block lists exceed 2^31 entries");
     }
     ns[d.leaves] = int(nb.size());
     nbr_start = upload(P, ns.data(), ns.size());
