@@ -129,6 +129,8 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-split M2L line")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the plan's launch autotuner at K states before timing")
     ap.add_argument("--near-variant", type=int, default=0,
                     help="pin the near-field kernel (codes of hip_plan_statistics; 0 = built-in default)")
     ap.add_argument("--no-other-paths", action="store_true",
@@ -157,6 +159,11 @@ def main() -> int:
     fmm = make_plan(hip, x, p, depth, hip.StaticPrecision.FLOAT32, hip.HipM2LMode.MFMA_BF16X3)
     if args.near_variant:
         fmm.set_launch_choices(0, args.near_variant)
+    if args.autotune:
+        fmm.autotune(K)
+        st = fmm.hip_plan_statistics
+        log(f"autotune: M2L tile shape {st['m2l_tile_shape']}, near-field variant {st['p2p_kernel_variant']} "
+            f"in {st['autotune_seconds']:.1f}s")
     log(f"plan setup: {time.perf_counter() - t0:.1f}s")
     stats = fmm.hip_plan_statistics
     moments = torch.from_numpy(moment_states(N, K, SEED)).to(dev)
