@@ -601,34 +601,35 @@ __device__ __forceinline__ void p2p_bf16_products(f32x4 (&part)[2], const uint4&
 
 // List-1 P2P on the matrix cores with bf16 operands
 // (v_mfma_f32_16x16x32_bf16), for FP32 plans with K a multiple of 32 (the
-// default there).  Fields are capped at 168 registers, three waves per SIMD
-// (5.8 ms against 6.1 ms uncapped at 1M points, p = 8, depth 5, K = 32); with
-// the potential (kPot: a fourth accumulator row, phi += f r . m from the
-// entries f r_c') two waves, where the cap would spill.  Both operands are split in three bf16 pieces (hi + mid + lo, 24
-// significant bits) and the six piece products whose orders sum to at most 2
-// are kept, D m ~ Dh mh + Dh mm + Dm mh + Dh ml + Dl mh + Dm mm: the dropped
-// ones are below 2^-26 |D m|, under the rounding of FP32 operands.  Against
-// the FP32 matrix-core product (16x16x4, 32 cycles for 2048 flops) the six
-// bf16 products issue 2.7x fewer matrix-core cycles per pair.
+// default there).  Both operands are split in three bf16 pieces (hi + mid +
+// lo, 24 significant bits) and the six piece products whose orders sum to at
+// most 2 are kept, D m ~ Dh mh + Dh mm + Dm mh + Dh ml + Dl mh + Dm mm: the
+// dropped ones are below 2^-26 |D m|, under the rounding of FP32 operands.
+// Against the FP32 matrix-core product (16x16x4, 32 cycles for 2048 flops)
+// the six bf16 products issue 2.7x fewer matrix-core cycles per pair.
 //
 // One wave owns 16 targets of one leaf and 32 states; the depth of a product
 // is four blocks of 8 sources from the list of the neighbourhood's blocks,
-// and lane (row i, group g) forms the pairs of target i with the 8 sources
-// of block g: their offsets r, then
-// the 6 distinct entries D_ab = 3 f r_a r_b / |r|^2 - delta_ab f with
-// f = |r|^-3 / (4 pi), each split into pieces for the products of both
-// components it couples.  Lane layout (CDNA4 16x16x32 bf16): A[i][k] in lane
-// i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8;
-// D[i][j] in lane j + 16 (i / 4), register i % 4.  The moments' pieces come
-// from split_moments_kernel in 8-source blocks, so a lane's B operand is one
-// 16-byte load; a group past the list's end reads the product's first
+// and lane (row i, group g) forms the pairs of target i with the 8 sources of
+// block g: their offsets r, then the 6 distinct entries
+// D_ab = 3 f r_a r_b / |r|^2 - delta_ab f with f = |r|^-3 / (4 pi), each
+// split into pieces for the products of both components it couples.  With
+// the potential (kPot) a fourth accumulator row takes phi += f r . m from the
+// entries f r_c'.  Lane layout (CDNA4 16x16x32 bf16): A[i][k] in lane
+// i + 16 (k / 8), element k % 8; B[k][j] in lane j + 16 (k / 8), element
+// k % 8; D[i][j] in lane j + 16 (i / 4), register i % 4.  The moments' pieces
+// come from split_moments_kernel in 8-source blocks, so a lane's B operand is
+// one 16-byte load; a group past the list's end reads the product's first
 // block, and its pairs, padding pairs and the identity pair have |r|^-1 = 0
 // and so zero entries.  Coincident pairs give NaN.  Blocked summation per
 // product (32 source slots) as in the FP32 kernels.
+//
+// Fields are capped at 168 registers, three waves per SIMD (4.7 ms at 1M
+// points, p = 8, depth 5, K = 32, against 7.4 ms for the FP32 kernel); the
+// potential form runs at two, where the cap would spill.
 template <bool kPot>
 __global__ void __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(kPot ? 2 : 3)))
-p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles,
-                                                                   int tile_count, int K) {
+p2p_bf16_kernel(const DevicePlan d, const int2* __restrict__ tiles, int tile_count, int K) {
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
   const int item = xcd_contiguous_block(int(blockIdx.x), int(gridDim.x)) * kP2PWaves + wave;
   if (item >= tile_count) return;
