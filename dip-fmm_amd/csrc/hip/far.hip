@@ -851,8 +851,9 @@ __global__ void __launch_bounds__(256) p2m_mfma_kernel(const DevicePlan d, int K
 // monomials), B = rows of the leaf local, one accumulator tile per output
 // component and 16-state half; the monomial operand is shared by the three
 // components.  One wave per 16-target tile (the near-field tiles), four per
-// workgroup; per wave LDS holds the axis powers and the monomial table
-// mono[g][t].  Writes out[k][user target][3] (and pot[k][user target]), or
+// workgroup; per wave LDS holds the axis powers x^i / i!, and each step forms
+// its monomial as the product of three of them (a monomial table in LDS kept
+// the kernel at 3 waves per SIMD).  Writes out[k][user target][3] (and pot[k][user target]), or
 // with kSorted the total fields over the near field in sorted order
 // Hn[t][3][K], for scatter_fields_kernel to unsort.
 template <bool kPot, bool kSorted>
@@ -862,10 +863,9 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
   const int n = d.n, p = d.p, NM = kPot ? d.n : d.nm, PW = p + 1;
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
   float* pw = reinterpret_cast<float*>(smem_raw) + wave * (48 * PW);       // [t][axis][i]
-  float* mono = reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + wave * (NM * 16);  // [g][t]
   // Basis tables shared by the workgroup: exponents of the NM monomials and
   // the raising table g -> g + e_c (so no step waits on a table load).
-  int4* ex_l = reinterpret_cast<int4*>(reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + 4 * NM * 16);
+  int4* ex_l = reinterpret_cast<int4*>(reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW);
   int4* up_l = ex_l + NM;
   for (int i = threadIdx.x; i < NM; i += 256) ex_l[i] = d.p2m_tab[n + i];
   for (int i = threadIdx.x; i < d.nm; i += 256) up_l[i] = d.l2p_up[i];
@@ -891,15 +891,13 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
     }
   }
   __syncthreads();  // axis powers and basis tables in LDS
-  for (int w = lane; w < NM * 16; w += 64) {
-    const int g = w >> 4, t = w & 15;
-    const int4 e = ex_l[g];
-    const float* pt = pw + t * 3 * PW;
-    mono[w] = pt[e.x] * pt[PW + e.y] * pt[2 * PW + e.z];
-  }
-  __syncthreads();
   if (!valid) return;
   const int col = lane & 15, q = lane >> 4;
+  const float* pt = pw + col * 3 * PW;  // this lane's target
+  auto mono = [&](int g) {
+    const int4 e = ex_l[g];
+    return pt[e.x] * pt[PW + e.y] * pt[2 * PW + e.z];
+  };
   const int kc0 = int(blockIdx.y) * 32;
   const int64_t node = level_offset(d.L) + tile.x;
   const float* Lb = static_cast<const float*>(d.Lc) + node * int64_t(n) * K + kc0 + col;
@@ -922,7 +920,7 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
       const int g = g00 + 4 * u + q;
       okv[u] = g < nm;
       const int gl = okv[u] ? g : 0;
-      av[u] = okv[u] ? mono[gl * 16 + col] : 0.0f;
+      av[u] = okv[u] ? mono(gl) : 0.0f;
       const int4 up = up_l[gl];
       const int rows[3] = {up.x, up.y, up.z};
 #pragma unroll
@@ -944,7 +942,7 @@ __global__ void __launch_bounds__(256) l2p_mfma_kernel(const DevicePlan d, float
       const int b = b0 + q;
       const bool ok = b < n;
       const int bl = ok ? b : 0;
-      const float a = ok ? mono[bl * 16 + col] : 0.0f;
+      const float a = ok ? mono(bl) : 0.0f;
       float bv[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) bv[h] = Lb[int64_t(bl) * K + 16 * h];
@@ -1375,8 +1373,7 @@ void launch_l2p_unsort(const DevicePlan& d, S* fields, S* potentials, int K, hip
   if (pot && d.Pn == nullptr) throw invalid_argument("near-field potential buffer is not allocated");
   if constexpr (sizeof(S) == 4) {
     const int NM = pot ? d.n : d.nm;
-    const size_t lds =
-        (size_t(4) * 48 * (d.p + 1) + size_t(4) * NM * 16) * sizeof(float) + size_t(NM + d.nm) * sizeof(int4);
+    const size_t lds = size_t(4) * 48 * (d.p + 1) * sizeof(float) + size_t(NM + d.nm) * sizeof(int4);
     if (K % 32 == 0 && d.l2p_up != nullptr && d.p2p_tile16_count > 0 && lds <= 65536) {
       const dim3 grid(unsigned((d.p2p_tile16_count + 3) / 4), unsigned(K / 32));
       // Fields only, whole 64-point tiles: the totals stay sorted and one
