@@ -1055,15 +1055,15 @@ void launch_p2m(const DevicePlan& d, int K, hipStream_t s) {
 // the six (c, j) = xx xy xz yy yz zz (a second raising table), as six 16 x 16
 // accumulator tiles per 16-state half; times the frame factor 1/W, written
 // to out[k][user target][6].  One wave per 16-target tile, four per
-// workgroup; four steps' loads are in flight together.
+// workgroup; four steps' loads are in flight together.  Each step forms its
+// monomial from three LDS axis powers, as the field's L2P does.
 __global__ void __launch_bounds__(256) l2p_grad_mfma_kernel(const DevicePlan d, float* __restrict__ out, int K,
                                                            int nm2, float scale) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = d.n, PW = max(d.p - 1, 1);  // axis powers 0..p-2
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6), lane = threadIdx.x & 63;
   float* pw = reinterpret_cast<float*>(smem_raw) + wave * (48 * PW);                 // [t][axis][i]
-  float* mono = reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + wave * (nm2 * 16);  // [g][t]
-  int4* ex_l = reinterpret_cast<int4*>(reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW + 4 * nm2 * 16);
+  int4* ex_l = reinterpret_cast<int4*>(reinterpret_cast<float*>(smem_raw) + 4 * 48 * PW);
   int* up_l = reinterpret_cast<int*>(ex_l + nm2);  // [g][8]
   for (int i = threadIdx.x; i < nm2; i += 256) ex_l[i] = d.p2m_tab[n + i];
   for (int i = threadIdx.x; i < nm2 * 8; i += 256) up_l[i] = d.grad_up[i];
@@ -1089,15 +1089,13 @@ __global__ void __launch_bounds__(256) l2p_grad_mfma_kernel(const DevicePlan d, 
     }
   }
   __syncthreads();  // axis powers and tables in LDS
-  for (int w = lane; w < nm2 * 16; w += 64) {
-    const int g = w >> 4, t = w & 15;
-    const int4 e = ex_l[g];
-    const float* pt = pw + t * 3 * PW;
-    mono[w] = pt[e.x] * pt[PW + e.y] * pt[2 * PW + e.z];
-  }
-  __syncthreads();
   if (!valid) return;
   const int col = lane & 15, q = lane >> 4;
+  const float* pt = pw + col * 3 * PW;  // this lane's target
+  auto mono = [&](int g) {
+    const int4 e = ex_l[g];
+    return pt[e.x] * pt[PW + e.y] * pt[2 * PW + e.z];
+  };
   const int kc0 = int(blockIdx.y) * 32;
   const int64_t node = level_offset(d.L) + tile.x;
   const float* Lb = static_cast<const float*>(d.Lc) + node * int64_t(n) * K + kc0 + col;
@@ -1113,7 +1111,7 @@ __global__ void __launch_bounds__(256) l2p_grad_mfma_kernel(const DevicePlan d, 
       const int g = g00 + 4 * u + q;
       okv[u] = g < nm2;
       const int gl = okv[u] ? g : 0;
-      av[u] = okv[u] ? mono[gl * 16 + col] : 0.0f;
+      av[u] = okv[u] ? mono(gl) : 0.0f;
 #pragma unroll
       for (int e = 0; e < 6; ++e)
 #pragma unroll
@@ -1152,8 +1150,8 @@ void launch_l2p_grad_unsort(const DevicePlan& d, S* gradients, int K, hipStream_
   if (d.Gn == nullptr || d.grad_tab == nullptr) throw invalid_argument("field-gradient buffers are not allocated");
   const int nm2 = d.p >= 2 ? coefficient_count(d.p - 2) : 0;
   if constexpr (sizeof(S) == 4) {
-    const size_t lds_m = (size_t(4) * 48 * std::max(d.p - 1, 1) + size_t(4) * nm2 * 16) * sizeof(float) +
-                         size_t(nm2) * sizeof(int4) + size_t(nm2) * 8 * sizeof(int);
+    const size_t lds_m = size_t(4) * 48 * std::max(d.p - 1, 1) * sizeof(float) + size_t(nm2) * sizeof(int4) +
+                         size_t(nm2) * 8 * sizeof(int);
     if (K % 32 == 0 && nm2 > 0 && d.grad_up != nullptr && d.p2p_tile16_count > 0 && lds_m <= 65536) {
       const dim3 grid(unsigned((d.p2p_tile16_count + 3) / 4), unsigned(K / 32));
       l2p_grad_mfma_kernel<<<grid, 256, lds_m, s>>>(d, gradients, K, nm2, float(1.0 / d.potential_scale));
