@@ -265,9 +265,10 @@ def test_near_field_variants_agree(hip, output):
         assert rel_l2(got[variant]["H"], got[1]["H"]) <= 2e-6
 
 
-@pytest.mark.parametrize("K", [32, 64])
-def test_bf16_three_piece_near_field_against_the_direct_sum(hip, K):
-    # Depth 1: every pair is a near pair, so the field is the P2P alone.  The
+@pytest.mark.parametrize("K,depth", [(32, 1), (64, 1), (32, 0)])
+def test_bf16_three_piece_near_field_against_the_direct_sum(hip, K, depth):
+    # Depth 1 (or 0, a single leaf): every pair is a near pair, so the field
+    # is the P2P alone.  The
     # three-piece bf16 products keep FP32 accuracy: against the FP64 direct
     # sum like the FP32 matrix-core kernel (both near 3e-7), and the two
     # kernels, whose FP32 roundings are independent, within the same 1e-6.
@@ -279,8 +280,8 @@ def test_bf16_three_piece_near_field_against_the_direct_sum(hip, K):
     x[7] = x[1000]
     ids = np.arange(n)
     ms = rng.normal(size=(K, n, 3))
-    announce("bf16 three-piece near field", K)
-    f = make(hip, x, x, 3, 1, 32, ids=ids)
+    announce("bf16 three-piece near field", K, depth)
+    f = make(hip, x, x, 3, depth, 32, ids=ids)
     got = {}
     for variant in (3, 4):
         f.set_launch_choices(0, variant)
