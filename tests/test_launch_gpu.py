@@ -265,6 +265,33 @@ def test_near_field_variants_agree(hip, output):
         assert rel_l2(got[variant]["H"], got[1]["H"]) <= 2e-6
 
 
+def test_bf16_near_field_with_a_shuffled_identity_map(hip):
+    # Targets are the sources in shuffled order, with the identity map: the
+    # bf16 kernel skips the pair (target t, source ids[t]) by its sorted
+    # index, as the FP32 kernel does; against the oracle's FP64 FMM with the
+    # same map.
+    rng = np.random.default_rng(986)
+    n, K = 6000, 32
+    src = rng.uniform(0, 1, size=(n, 3))
+    perm = rng.permutation(n)
+    tgt = src[perm]
+    ms = rng.normal(size=(K, n, 3))
+    announce("bf16 near field, shuffled identity map")
+    f = make(hip, src, tgt, 4, 3, 32, ids=perm)
+    got = {}
+    for variant in (3, 4):
+        f.set_launch_choices(0, variant)
+        got[variant] = f.evaluate_batch(ms)
+    ref = O.CpuFmm(src, tgt, 4, 3, 64, fixed_ids=perm)
+    for k in (0, K - 1):
+        want = ref.evaluate(ms[k], perm)["H"]
+        for variant in (3, 4):
+            e = rel_l2(got[variant][k], want)
+            print(f"  state {k} variant {variant}: vs FP64 FMM {e:.3e}", flush=True)
+            assert e <= 1e-5
+    assert rel_l2(got[4], got[3]) <= 1e-6
+
+
 @pytest.mark.parametrize("K,depth", [(32, 1), (64, 1), (32, 0)])
 def test_bf16_three_piece_near_field_against_the_direct_sum(hip, K, depth):
     # Depth 1 (or 0, a single leaf): every pair is a near pair, so the field
